@@ -1,0 +1,195 @@
+/*
+ * shotsweep.h - C ABI of libshotsweep.so, the gfx950 HIP implementation of the
+ * batched 2-D constant-density acoustic finite-difference simulator, its exact
+ * discrete adjoint and the per-shot RTM imaging condition.
+ *
+ * Reference interface replaced: the PyTorch op sequences inside
+ * openfwi/physics/gpu.py (class Sim).  Every entry point names the lines it
+ * replaces.  The Python module shotsweep_amd.gpu keeps the reference's call
+ * surface (Sim, lap, lap_adjoint, ...) and calls these functions through
+ * ctypes on raw device pointers.
+ *
+ * Conventions
+ *  - All arrays are device pointers, C-contiguous, in the element type named
+ *    by the suffix (_f32 = float, _f64 = double).  The caller allocates every
+ *    buffer; the library never frees caller memory and keeps no pointer after
+ *    a call returns.
+ *  - Padded grid: H = nz + 2*pad rows, W = nx + 2*pad columns.  A
+ *    "propagation" is one (row b, shot s) pair; there are nb*ns of them.
+ *  - Calls are asynchronous and ordered on `stream` (a hipStream_t; NULL is the
+ *    default stream).  They return 0 on success and a negative code on error;
+ *    ss_last_error() then gives a message (thread-local).  No C++ exception
+ *    crosses the ABI.
+ *  - Arithmetic reproduces the reference's FP32/FP64 operation order (no
+ *    contraction of separately rounded operations into FMAs, correctly
+ *    rounded division), so results equal the reference CPU path bit for bit
+ *    except where noted (the reduction order of sums).
+ */
+#ifndef SHOTSWEEP_H
+#define SHOTSWEEP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SS_MAX_SHOTS 32
+
+/* Error codes */
+#define SS_OK 0
+#define SS_EINVAL (-1)   /* bad argument (shape, range, null pointer) */
+#define SS_EHIP (-2)     /* HIP runtime error */
+#define SS_EUNSUP (-3)   /* requested implementation cannot run this shape */
+
+/* Implementation selector for the time-loop entry points. */
+#define SS_IMPL_AUTO 0      /* resident when the plane fits the LDS, else stepwise */
+#define SS_IMPL_STEPWISE 1  /* one launch per time step, planes in HBM (FP32/FP64, any size) */
+#define SS_IMPL_RESIDENT 2  /* one persistent launch per step range, plane in LDS (FP32) */
+
+/*
+ * Geometry of one batch of propagations (openfwi/physics/constants.py:4-7,
+ * gpu.py:14, generalised to rectangular grids and any shot count).
+ */
+typedef struct ss_geom {
+  int32_t nb;                 /* velocity rows (models x backgrounds), gpu.py:126 len(v) */
+  int32_t ns;                 /* shots per row, gpu.py:95 (5) */
+  int32_t nz, nx;             /* model grid, constants.py:6 (70 x 70) */
+  int32_t pad;                /* absorbing-layer width in cells, gpu.py:77 (40) */
+  int32_t nt;                 /* time steps, constants.py:6 (1000) */
+  int32_t row;                /* model row of sources and receivers, constants.py:7 (row 1) */
+  int32_t src[SS_MAX_SHOTS];  /* source columns in model coordinates, gpu.py:14 */
+} ss_geom;
+
+/* Message of the last failed call on this thread ("" if none). */
+const char *ss_last_error(void);
+
+/* ABI version (incremented on any signature change). */
+int ss_version(void);
+
+/* Largest padded plane (H*W cells) the resident FP32 kernels hold on one CU. */
+int64_t ss_resident_max_cells(void);
+
+/*
+ * Forward time loop over steps [t0, t1), replacing gpu.py:111-115 (Sim._step)
+ * and the loop gpu.py:126-135 (Sim.forward), including lap (gpu.py:33-37).
+ *
+ *   c2      (nb, H, W)   vp^2 (dt/dx)^2, formed in float64 and rounded (gpu.py:107)
+ *   A, B    (H, W)       sponge coefficients 1/(1+a), 1-a (gpu.py:85-90)
+ *   w       (nt)         wavelet * 1e3 (gpu.py:94)
+ *   amp     (nb, ns)     (v[row, src]/2000)^2 (gpu.py:108)
+ *   p0, p1  (nb, ns, H, W) state before step t0 (in) and after step t1-1 (out):
+ *                          p0 <- field at t1-1, p1 <- field at t1
+ *   gathers (nb, ns, nt, nx) or NULL: receiver samples of steps [t0, t1) (gpu.py:134)
+ *   lap_hist (nb, ns, t1-t0, H, W) or NULL: Laplacian of each step (gpu.py:112, 172-175)
+ *   ckpt    (nb, ns, ceil((t1-t0)/ckpt_every), 2, H, W) or NULL: state (p0, p1)
+ *           before every step t with (t - t0) % ckpt_every == 0 (gpu.py:131-132)
+ */
+int ss_forward_f32(const ss_geom *g, int t0, int t1, const float *c2, const float *A, const float *B,
+                   const float *w, const float *amp, float *p0, float *p1, float *gathers,
+                   float *lap_hist, float *ckpt, int ckpt_every, int impl, void *stream);
+int ss_forward_f64(const ss_geom *g, int t0, int t1, const double *c2, const double *A, const double *B,
+                   const double *w, const double *amp, double *p0, double *p1, double *gathers,
+                   double *lap_hist, double *ckpt, int ckpt_every, int impl, void *stream);
+
+/*
+ * Adjoint time loop over steps t1-1 down to t0, replacing the reverse loop
+ * gpu.py:177-184 (adjoint recursion with lap_adjoint gpu.py:40-53, receiver
+ * injection, imaging condition, source-amplitude term, illumination).
+ *
+ *   sd       (nb, ns)   source-amplitude derivative 2 v[row,src]/2000^2 (gpu.py:168)
+ *   res      (nb, ns, nt, nx) muted residual (gpu.py:158-160)
+ *   lap_hist (nb, ns, t1-t0, H, W) Laplacians of the forward field at steps [t0, t1)
+ *   m1, m2   (nb, ns, H, W) adjoint state (in at t1, out at t0): m1 <- mu(t0), m2 <- mu(t0+1)
+ *   gacc     (nb, ns, H, W) running sum of A*mu*L (gpu.py:181)
+ *   illum    (nb, ns, nz, nx) running sum of L^2 over the model cells (gpu.py:183)
+ *   src_grad (nb, ns)  running source-amplitude term (gpu.py:182)
+ */
+int ss_adjoint_f32(const ss_geom *g, int t0, int t1, const float *c2, const float *A, const float *B,
+                   const float *w, const float *sd, const float *res, const float *lap_hist,
+                   float *m1, float *m2, float *gacc, float *illum, float *src_grad, int impl, void *stream);
+int ss_adjoint_f64(const ss_geom *g, int t0, int t1, const double *c2, const double *A, const double *B,
+                   const double *w, const double *sd, const double *res, const double *lap_hist,
+                   double *m1, double *m2, double *gacc, double *illum, double *src_grad, int impl,
+                   void *stream);
+
+/*
+ * Residual with the direct-wave mute, replacing gpu.py:142-150 and 158-160:
+ *   res = (sim - obs) * mute  (mute omitted when v_mute is NULL)
+ *   sim      (nb, ns, nt, nx)
+ *   obs      (nb / obs_repeat, ns, nt, nx): row b of sim uses row b / obs_repeat of obs
+ *            (obs_repeat = 1 reproduces the reference's explicit np.repeat)
+ *   v_mute   (nb) mute velocity per row, or NULL
+ *   dt_s, dx_m, t_mute, ramp: dt (s), dx (m), mute delay (s), ramp length (s, 0.03)
+ *   res      (nb, ns, nt, nx) output (may alias sim)
+ * ss_mute_weights_* writes the weights alone, (nb, ns, nt, nx).
+ */
+int ss_residual_f32(const ss_geom *g, const float *sim, const float *obs, int obs_repeat,
+                    const float *v_mute, double dt_s, double dx_m, double t_mute, double ramp,
+                    float *res, void *stream);
+int ss_residual_f64(const ss_geom *g, const double *sim, const double *obs, int obs_repeat,
+                    const double *v_mute, double dt_s, double dx_m, double t_mute, double ramp,
+                    double *res, void *stream);
+int ss_mute_weights_f32(const ss_geom *g, const float *v_mute, double dt_s, double dx_m, double t_mute,
+                        double ramp, float *out, void *stream);
+int ss_mute_weights_f64(const ss_geom *g, const double *v_mute, double dt_s, double dx_m, double t_mute,
+                        double ramp, double *out, void *stream);
+
+/*
+ * Gradient epilogue, replacing gpu.py:185-187 and fold_edge_padding
+ * gpu.py:56-67:  grad = fold(gacc * 2 * vp * k) ; grad[row, src] += src_grad
+ *   v        (nb, nz, nx) model velocity in the working type (vp = edge padding of v)
+ *   k        (dt/dx)^2 rounded to the working type
+ *   grad     (nb, ns, nz, nx) output
+ * Sums over the padding strips run in ascending index order; the reference's
+ * torch reduction order may differ in the last bit.
+ */
+int ss_gradient_f32(const ss_geom *g, const float *gacc, const float *v, const float *src_grad, float k,
+                    float *grad, void *stream);
+int ss_gradient_f64(const ss_geom *g, const double *gacc, const double *v, const double *src_grad, double k,
+                    double *grad, void *stream);
+
+/*
+ * Image epilogue, replacing gpu.py:206-209:
+ *   den = illum + eps * max(illum) per propagation ; img = -grad / max(den, tiny)
+ *   if laplacian: img = -lap(replicate_pad2(img)) on the model grid
+ *   grad, illum (nb, ns, nz, nx) ; img (nb, ns, nz, nx) output
+ */
+int ss_image_f32(const ss_geom *g, const float *grad, const float *illum, float eps, int laplacian,
+                 float *img, void *stream);
+int ss_image_f64(const ss_geom *g, const double *grad, const double *illum, double eps, int laplacian,
+                 double *img, void *stream);
+
+/*
+ * Tangent-linear (Born) forward over steps [t0, t1): no reference counterpart
+ * (the reference has no linearised operator); it is the exact derivative of
+ * ss_forward_* with respect to velocity, used for the dot-product test.
+ *   dc2   (nb, H, W)  2 vp dvp (dt/dx)^2 ;  damp (nb, ns) 2 v dv / 2000^2 at the source
+ *   q0,q1 (nb, ns, H, W) perturbation state ; p0,p1 background state (as ss_forward_*)
+ *   born  (nb, ns, nt, nx) receiver samples of the perturbation
+ */
+int ss_born_f32(const ss_geom *g, int t0, int t1, const float *c2, const float *dc2, const float *A,
+                const float *B, const float *w, const float *amp, const float *damp, float *p0, float *p1,
+                float *q0, float *q1, float *born, void *stream);
+int ss_born_f64(const ss_geom *g, int t0, int t1, const double *c2, const double *dc2, const double *A,
+                const double *B, const double *w, const double *amp, const double *damp, double *p0,
+                double *p1, double *q0, double *q1, double *born, void *stream);
+
+/*
+ * Free functions of the reference module, on n planes of H x W:
+ *   ss_lap_*                gpu.py:33-37 lap (zero on the two outer cells)
+ *   ss_lap_adjoint_*        gpu.py:40-53 lap_adjoint (exact transpose of lap)
+ *   ss_fold_edge_padding_*  gpu.py:56-67 fold_edge_padding: g (n, nz+2pad, nx+2pad) -> out (n, nz, nx)
+ */
+int ss_lap_f32(int64_t n, int H, int W, const float *in, float *out, void *stream);
+int ss_lap_f64(int64_t n, int H, int W, const double *in, double *out, void *stream);
+int ss_lap_adjoint_f32(int64_t n, int H, int W, const float *in, float *out, void *stream);
+int ss_lap_adjoint_f64(int64_t n, int H, int W, const double *in, double *out, void *stream);
+int ss_fold_edge_padding_f32(int64_t n, int nz, int nx, int pad, const float *g, float *out, void *stream);
+int ss_fold_edge_padding_f64(int64_t n, int nz, int nx, int pad, const double *g, double *out, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SHOTSWEEP_H */

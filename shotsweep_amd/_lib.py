@@ -1,0 +1,112 @@
+"""ctypes binding of libshotsweep.so (declarations in include/shotsweep.h).
+
+The library is loaded from this package directory only; there is no fallback.
+A missing or stale library raises RuntimeError telling how to build it.
+"""
+import ctypes
+from pathlib import Path
+
+import torch
+
+LIB_PATH = Path(__file__).resolve().parent / 'libshotsweep.so'
+MAX_SHOTS = 32
+IMPL_AUTO, IMPL_STEPWISE, IMPL_RESIDENT = 0, 1, 2
+ABI_VERSION = 1
+
+_vp = ctypes.c_void_p
+_i = ctypes.c_int
+_i64 = ctypes.c_int64
+_d = ctypes.c_double
+
+
+class SSGeom(ctypes.Structure):
+    """Mirror of ss_geom."""
+    _fields_ = [('nb', ctypes.c_int32), ('ns', ctypes.c_int32), ('nz', ctypes.c_int32), ('nx', ctypes.c_int32),
+                ('pad', ctypes.c_int32), ('nt', ctypes.c_int32), ('row', ctypes.c_int32),
+                ('src', ctypes.c_int32 * MAX_SHOTS)]
+
+
+def make_geom(nb, ns, nz, nx, pad, nt, row, src):
+    if len(src) != ns or ns > MAX_SHOTS:
+        raise ValueError(f'need one source column per shot (ns <= {MAX_SHOTS})')
+    g = SSGeom(nb, ns, nz, nx, pad, nt, row)
+    for k, c in enumerate(src):
+        g.src[k] = int(c)
+    return g
+
+
+# name -> argtypes (all functions return int unless listed in _RESTYPES)
+def _sig(real):
+    R = real  # scalar type of the working precision for by-value reals
+    geom = ctypes.POINTER(SSGeom)
+    return {
+        'forward': [geom, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _vp],
+        'adjoint': [geom, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _vp],
+        'residual': [geom, _vp, _vp, _i, _vp, _d, _d, _d, _d, _vp, _vp],
+        'mute_weights': [geom, _vp, _d, _d, _d, _d, _vp, _vp],
+        'gradient': [geom, _vp, _vp, _vp, R, _vp, _vp],
+        'image': [geom, _vp, _vp, R, _i, _vp, _vp],
+        'born': [geom, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+        'lap': [_i64, _i, _i, _vp, _vp, _vp],
+        'lap_adjoint': [_i64, _i, _i, _vp, _vp, _vp],
+        'fold_edge_padding': [_i64, _i, _i, _i, _vp, _vp, _vp],
+    }
+
+
+EXPORTS = ['ss_last_error', 'ss_version', 'ss_resident_max_cells'] + [
+    f'ss_{name}_{sfx}' for sfx in ('f32', 'f64') for name in _sig(ctypes.c_float)]
+
+_LIB = None
+
+
+def load():
+    """Load the library once; raise RuntimeError when it is missing."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not LIB_PATH.exists():
+        raise RuntimeError(f'{LIB_PATH} is not built; run `python -m shotsweep_amd.build` '
+                           '(shotsweep_amd has no CPU fallback)')
+    lib = ctypes.CDLL(str(LIB_PATH))
+    lib.ss_last_error.restype = ctypes.c_char_p
+    lib.ss_last_error.argtypes = []
+    lib.ss_version.restype = ctypes.c_int
+    lib.ss_resident_max_cells.restype = ctypes.c_int64
+    for sfx, real in (('f32', ctypes.c_float), ('f64', ctypes.c_double)):
+        for name, args in _sig(real).items():
+            fn = getattr(lib, f'ss_{name}_{sfx}')
+            fn.argtypes = args
+            fn.restype = ctypes.c_int
+    if lib.ss_version() != ABI_VERSION:
+        raise RuntimeError(f'libshotsweep ABI {lib.ss_version()} != expected {ABI_VERSION}; rebuild it')
+    _LIB = lib
+    return lib
+
+
+def fn(name, dtype):
+    """Entry point `ss_<name>_<f32|f64>` for a torch dtype."""
+    sfx = {torch.float32: 'f32', torch.float64: 'f64'}.get(dtype)
+    if sfx is None:
+        raise TypeError(f'shotsweep kernels support float32 and float64, not {dtype}')
+    return getattr(load(), f'ss_{name}_{sfx}')
+
+
+def check(rc, what=''):
+    if rc != 0:
+        msg = load().ss_last_error().decode(errors='replace')
+        raise RuntimeError(f'shotsweep {what} failed ({rc}): {msg}')
+
+
+def ptr(t):
+    """Device pointer of a tensor (None -> NULL)."""
+    if t is None:
+        return None
+    if not t.is_cuda:
+        raise ValueError('shotsweep kernels take device tensors')
+    if not t.is_contiguous():
+        raise ValueError('shotsweep kernels take contiguous tensors')
+    return t.data_ptr()
+
+
+def stream_handle(device):
+    return torch.cuda.current_stream(device).cuda_stream

@@ -1,0 +1,46 @@
+// Internal declarations shared by the kernel translation units and the ABI.
+#pragma once
+
+#include "ss_common.h"
+
+namespace ss {
+
+template <typename T>
+int forward_stepwise(const Geom &g, int t0, int t1, const T *c2, const T *A, const T *B, const T *w, const T *amp,
+                     T *p0, T *p1, T *gathers, T *hist, T *ckpt, int ckpt_every, hipStream_t st);
+
+template <typename T>
+int adjoint_stepwise(const Geom &g, int t0, int t1, const T *c2, const T *A, const T *B, const T *w, const T *sd,
+                     const T *res, const T *hist, T *m1, T *m2, T *gacc, T *illum, T *src_grad, hipStream_t st);
+
+template <typename T>
+int born_stepwise(const Geom &g, int t0, int t1, const T *c2, const T *dc2, const T *A, const T *B, const T *w,
+                  const T *amp, const T *damp, T *p0, T *p1, T *q0, T *q1, T *born, hipStream_t st);
+
+template <typename T>
+int residual(const Geom &g, const T *sim, const T *obs, int obs_repeat, const T *v_mute, double dt, double dx,
+             double t_mute, double ramp, T *res, int weights_only, hipStream_t st);
+
+template <typename T>
+int gradient_epilogue(const Geom &g, const T *gacc, const T *v, const T *src_grad, T k, T *grad, hipStream_t st);
+
+template <typename T>
+int image_epilogue(const Geom &g, const T *grad, const T *illum, T eps, int laplacian, T *img, hipStream_t st);
+
+template <typename T>
+int lap_op(int64_t n, int H, int W, const T *in, T *out, int adjoint, hipStream_t st);
+
+template <typename T>
+int fold_op(int64_t n, int nz, int nx, int pad, const T *g, T *out, hipStream_t st);
+
+// Resident FP32 kernels (plane held in LDS, one workgroup per propagation).
+int64_t resident_max_cells();
+bool resident_fits(const Geom &g);
+int forward_resident(const Geom &g, int t0, int t1, const float *c2, const float *A, const float *B, const float *w,
+                     const float *amp, float *p0, float *p1, float *gathers, float *hist, float *ckpt,
+                     int ckpt_every, hipStream_t st);
+int adjoint_resident(const Geom &g, int t0, int t1, const float *c2, const float *A, const float *B, const float *w,
+                     const float *sd, const float *res, const float *hist, float *m1, float *m2, float *gacc,
+                     float *illum, float *src_grad, hipStream_t st);
+
+}  // namespace ss

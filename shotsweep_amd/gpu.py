@@ -1,0 +1,433 @@
+"""Batched finite-difference simulation and exact discrete-adjoint images on MI355X.
+
+Drop-in for the reference module ``openfwi/physics/gpu.py``: the same names
+(``Sim``, ``lap``, ``lap_adjoint``, ``fold_edge_padding``,
+``imaging_memory_bytes``, ``calibrate``, ``resolve_device``, ``DEV``, ``SRC``,
+``RICKER``, ``torch``), signatures, defaults, return types and error
+messages.  The time loops run in hand-written gfx950 HIP kernels
+(``libshotsweep.so``, called through ctypes on raw device pointers on the
+current HIP stream); this module only forms the float64 coefficients on the
+host exactly as the reference does, moves them to the device and sequences the
+kernels.  There is no CPU path: a ``Sim`` on a non-GPU device raises.
+
+Numerics: the kernels reproduce the reference's operation order, so FP32 and
+FP64 gathers, gradients and images match the reference CPU path to the last
+bit except for the reduction order of the padding fold.
+
+Beyond the reference: ``Sim(..., geom=Acquisition(...))`` runs other grid
+sizes, shot counts and step counts; ``history`` selects whether the adjoint
+reads a stored Laplacian history ('full') or recomputes checkpointed segments
+('checkpoint', the reference's scheme); 'auto' stores the full history when it
+fits the device memory budget.
+"""
+import math
+
+import numpy as np
+import torch
+
+from . import _lib
+from .constants import DX, DT, NX, NZ, NT, F0, SRC_X, OPENFWI, Acquisition
+
+DEV = torch.device('cuda' if torch.cuda.is_available() else 'cpu')
+SRC = list(OPENFWI.src)
+RICKER = OPENFWI.ricker()
+
+__all__ = ['DEV', 'SRC', 'RICKER', 'Sim', 'lap', 'lap_adjoint', 'fold_edge_padding', 'imaging_memory_bytes',
+           'calibrate', 'resolve_device', 'torch', 'DX', 'DT', 'NX', 'NZ', 'NT', 'F0', 'SRC_X']
+
+
+def resolve_device(name='auto'):
+    """Device selection with the reference's errors (gpu.py:20-30)."""
+    if name == 'auto':
+        return torch.device('cuda' if torch.cuda.is_available() else 'cpu')
+    device = torch.device(name)
+    if device.type == 'cuda' and not torch.cuda.is_available():
+        raise RuntimeError('CUDA was requested, but this PyTorch installation has no available CUDA device')
+    if device.type == 'mps':
+        raise RuntimeError('MPS was requested, but no MPS device is available')
+    if device.type not in ('cpu', 'cuda', 'mps'):
+        raise ValueError(f'unsupported device: {name}')
+    return device
+
+
+def _require_gpu(t, what):
+    if not t.is_cuda:
+        raise RuntimeError(f'{what}: shotsweep_amd runs on a ROCm GPU; got a {t.device} tensor')
+
+
+def _planes(p):
+    """(..., H, W) tensor -> contiguous (n, H, W) view/copy and n."""
+    p = p.contiguous()
+    H, W = p.shape[-2:]
+    n = p.numel() // (H * W) if p.numel() else 0
+    return p, n, H, W
+
+
+def lap(p):
+    """Fourth-order Laplacian times dx^2, zero on the two outer cells (gpu.py:33-37)."""
+    _require_gpu(p, 'lap')
+    p, n, H, W = _planes(p)
+    out = torch.empty_like(p)
+    _lib.check(_lib.fn('lap', p.dtype)(n, H, W, _lib.ptr(p), _lib.ptr(out), _lib.stream_handle(p.device)), 'lap')
+    return out
+
+
+def lap_adjoint(p):
+    """Transpose of lap, including its two-cell zero-output boundary (gpu.py:40-53)."""
+    _require_gpu(p, 'lap_adjoint')
+    p, n, H, W = _planes(p)
+    out = torch.empty_like(p)
+    _lib.check(_lib.fn('lap_adjoint', p.dtype)(n, H, W, _lib.ptr(p), _lib.ptr(out), _lib.stream_handle(p.device)),
+               'lap_adjoint')
+    return out
+
+
+def fold_edge_padding(g, pad):
+    """Transpose of np.pad(..., mode='edge') for the last two axes (gpu.py:56-67)."""
+    _require_gpu(g, 'fold_edge_padding')
+    g, n, H, W = _planes(g)
+    nz, nx = H - 2 * pad, W - 2 * pad
+    out = torch.empty(*g.shape[:-2], nz, nx, dtype=g.dtype, device=g.device)
+    _lib.check(_lib.fn('fold_edge_padding', g.dtype)(n, nz, nx, pad, _lib.ptr(g), _lib.ptr(out),
+                                                      _lib.stream_handle(g.device)), 'fold_edge_padding')
+    return out
+
+
+def imaging_memory_bytes(batch, pad=40, checkpoint_interval=40, bytes_per_value=4):
+    """Conservative tensor-storage estimate of the reference scheme (gpu.py:70-73).
+
+    Kept identical so that callers size batches as before; ``Sim.workspace_bytes``
+    gives this implementation's own device footprint."""
+    fields = 2 * math.ceil(NT / checkpoint_interval) + checkpoint_interval + 24
+    return int(batch * 5 * bytes_per_value * ((NZ + 2 * pad) * (NX + 2 * pad) * fields + 3 * NT * NX))
+
+
+class Sim:
+    """Batched acoustic FD simulator, exact adjoint and RTM images (gpu.py:76-210)."""
+
+    def __init__(self, wavelet=None, pad=40, R=1e-5, checkpoint_interval=40,
+                 device=None, dtype=torch.float32, *, geom=None, history='auto', max_history_bytes=None,
+                 impl=_lib.IMPL_AUTO):
+        if pad < 2 or checkpoint_interval < 1:
+            raise ValueError('pad must be >= 2 and checkpoint_interval must be positive')
+        if history not in ('auto', 'full', 'checkpoint'):
+            raise ValueError("history must be 'auto', 'full' or 'checkpoint'")
+        self.geom = OPENFWI if geom is None else geom
+        self.pad = pad
+        self.checkpoint_interval = checkpoint_interval
+        self.device = DEV if device is None else resolve_device(str(device))
+        if self.device.type != 'cuda':
+            raise RuntimeError(f'shotsweep_amd.gpu.Sim runs its kernels on a ROCm GPU; device {self.device} '
+                               'is not supported (there is no CPU path)')
+        if dtype not in (torch.float32, torch.float64):
+            raise ValueError('dtype must be torch.float32 or torch.float64')
+        self.dtype = dtype
+        self.history = history
+        self.max_history_bytes = max_history_bytes
+        self.impl = impl
+        g = self.geom
+        self.H, self.W = g.nz + 2 * pad, g.nx + 2 * pad
+        A, B = self._damping(R)
+        self.A = self._tensor(A)
+        self.B = self._tensor(B)
+        self.wav = np.asarray(g.ricker() if wavelet is None else wavelet)
+        if self.wav.shape != (g.nt,) or not np.isfinite(self.wav).all() or not np.any(self.wav):
+            raise ValueError(f'wavelet must contain {g.nt} finite samples and nonzero energy')
+        self.w = self._tensor(self.wav * 1e3)
+        self.si = torch.arange(g.ns, device=self.device)
+        self.sx = torch.tensor(list(g.src), device=self.device) + pad
+        self._geom_c = None
+        self._ws = {}
+
+    # ---------------------------------------------------------------- setup
+    def _damping(self, R):
+        """Quadratic sponge, float64 (gpu.py:85-90), one profile per axis."""
+        pad, g = self.pad, self.geom
+
+        def depth(n):
+            i = np.arange(n)
+            return np.clip(np.maximum(pad - i, i - (n - 1 - pad)), 0, None) / pad
+
+        dz, dx = depth(self.H), depth(self.W)
+        a = 3 * 4500 / (2 * pad * g.dx) * np.log(1 / R) * (dz[:, None] ** 2 + dx[None, :] ** 2) * g.dt / 2
+        return 1 / (1 + a), 1 - a
+
+    def _tensor(self, value):
+        return torch.as_tensor(value, dtype=self.dtype, device=self.device)
+
+    def _host(self, value):
+        return torch.as_tensor(value, dtype=self.dtype)
+
+    def _geom(self, nb):
+        g = self.geom
+        return _lib.make_geom(nb, g.ns, g.nz, g.nx, self.pad, g.nt, g.row, g.src)
+
+    def _setup(self, v):
+        """c2 (B,1,H,W) and amp (B,ns) (gpu.py:101-109)."""
+        g = self.geom
+        v = np.asarray(v, np.float64)
+        if v.ndim != 3 or v.shape[1:] != (g.nz, g.nx) or not np.isfinite(v).all() or np.any(v <= 0):
+            raise ValueError(f'velocities must be finite positive (B, {g.nz}, {g.nx}) arrays')
+        p = self.pad
+        vp = np.pad(v, ((0, 0), (p, p), (p, p)), mode='edge')
+        c2 = self._tensor(vp ** 2 * (g.dt / g.dx) ** 2)[:, None]
+        amp = self._tensor((v[:, g.row, list(g.src)] / 2000.0) ** 2)
+        return c2, amp
+
+    def _source_derivative(self, v):
+        """2 v[row, src] / 2000^2 in the working dtype (gpu.py:168), formed on the host
+        with the reference's CPU division."""
+        g = self.geom
+        sd = 2 * self._host(np.asarray(v, np.float64)[:, g.row, list(g.src)]) / 2000.0 ** 2
+        return sd.to(self.device)
+
+    def _scratch(self, name, shape, zero=False):
+        t = self._ws.get(name)
+        n = int(np.prod(shape))
+        if t is None or t.numel() < n:
+            self._ws.pop(name, None)
+            t = torch.empty(n, dtype=self.dtype, device=self.device)
+            self._ws[name] = t
+        out = t[:n].view(*shape)
+        if zero:
+            out.zero_()
+        return out
+
+    def release(self):
+        """Free cached device workspaces."""
+        self._ws.clear()
+
+    # ------------------------------------------------------------- kernels
+    def _stream(self):
+        return _lib.stream_handle(self.device)
+
+    def _forward(self, geom, t0, t1, c2, amp, p0, p1, gathers=None, hist=None, ckpt=None, every=1, impl=None):
+        f = _lib.fn('forward', self.dtype)
+        _lib.check(f(geom, t0, t1, _lib.ptr(c2), _lib.ptr(self.A), _lib.ptr(self.B), _lib.ptr(self.w),
+                     _lib.ptr(amp), _lib.ptr(p0), _lib.ptr(p1), _lib.ptr(gathers), _lib.ptr(hist), _lib.ptr(ckpt),
+                     every, self.impl if impl is None else impl, self._stream()), 'forward')
+
+    def _adjoint(self, geom, t0, t1, c2, sd, res, hist, m1, m2, gacc, illum, sg, impl=None):
+        f = _lib.fn('adjoint', self.dtype)
+        _lib.check(f(geom, t0, t1, _lib.ptr(c2), _lib.ptr(self.A), _lib.ptr(self.B), _lib.ptr(self.w),
+                     _lib.ptr(sd), _lib.ptr(res), _lib.ptr(hist), _lib.ptr(m1), _lib.ptr(m2), _lib.ptr(gacc),
+                     _lib.ptr(illum), _lib.ptr(sg), self.impl if impl is None else impl, self._stream()), 'adjoint')
+
+    def _step(self, p0, p1, c2, amp, it):
+        """One damped leapfrog step; returns (p2, L) (gpu.py:111-115)."""
+        nb = p1.shape[0]
+        a, b = p0.contiguous().clone(), p1.contiguous().clone()
+        L = torch.empty(nb, self.geom.ns, 1, self.H, self.W, dtype=self.dtype, device=self.device)
+        self._forward(self._geom(nb), it, it + 1, c2.contiguous(), amp.contiguous(), a, b, hist=L,
+                      impl=_lib.IMPL_STEPWISE)
+        return b, L[:, :, 0]
+
+    def forward(self, v, store=False):
+        """Return gathers, optional wavefield checkpoints, and squared Courant numbers.
+
+        Checkpoints hold the states (p0, p1) before every checkpoint_interval-th
+        step, as views into one device buffer (gpu.py:117-136)."""
+        g = self.geom
+        c2, amp = self._setup(v)
+        nb = len(c2)
+        shape = (nb, g.ns, self.H, self.W)
+        p0 = torch.zeros(shape, dtype=self.dtype, device=self.device)
+        p1 = torch.zeros_like(p0)
+        out = torch.empty(nb, g.ns, g.nt, g.nx, dtype=self.dtype, device=self.device)
+        ci = self.checkpoint_interval
+        if store:
+            nck = -(-g.nt // ci)
+            ck = torch.empty(nb, g.ns, nck, 2, self.H, self.W, dtype=self.dtype, device=self.device)
+            self._forward(self._geom(nb), 0, g.nt, c2, amp, p0, p1, gathers=out, ckpt=ck, every=ci)
+            checkpoints = {k * ci: (ck[:, :, k, 0], ck[:, :, k, 1]) for k in range(nck)}
+        else:
+            self._forward(self._geom(nb), 0, g.nt, c2, amp, p0, p1, gathers=out)
+            checkpoints = None
+        return out, checkpoints, c2
+
+    @torch.no_grad()
+    def gathers(self, v, chunk=20):
+        return np.concatenate([self.forward(v[i:i + chunk])[0].cpu().numpy() for i in range(0, len(v), chunk)])
+
+    def _check_mute(self, v_mute):
+        v_mute = np.asarray(v_mute)
+        if not np.isfinite(v_mute).all() or np.any(v_mute <= 0):
+            raise ValueError('mute velocities must be finite and positive')
+        return self._tensor(v_mute).reshape(-1).contiguous()
+
+    def mute_weights(self, v_mute, t_mute=0.12):
+        """Direct-wave cosine mute, shape (len(v_mute), ns, nt, nx) (gpu.py:142-150)."""
+        g = self.geom
+        vm = self._check_mute(v_mute)
+        out = torch.empty(len(vm), g.ns, g.nt, g.nx, dtype=self.dtype, device=self.device)
+        f = _lib.fn('mute_weights', self.dtype)
+        _lib.check(f(self._geom(len(vm)), _lib.ptr(vm), g.dt, g.dx, t_mute, 0.03, _lib.ptr(out), self._stream()),
+                   'mute_weights')
+        return out
+
+    # --------------------------------------------------------- memory plan
+    def workspace_bytes(self, nb, history=None):
+        """Device bytes of one gradient/images call on nb rows under a history mode."""
+        g = self.geom
+        itemsize = torch.empty((), dtype=self.dtype).element_size()
+        plane = g.ns * self.H * self.W * itemsize
+        seg = self._segment(nb, history)
+        traces = g.ns * g.nt * g.nx * itemsize
+        hist = plane * seg
+        ckpt = 0 if seg >= g.nt else plane * 2 * (-(-g.nt // seg))
+        return nb * (hist + ckpt + 5 * plane + 2 * traces)
+
+    def _segment(self, nb, history=None):
+        """Steps per adjoint segment: nt (whole history stored) or the checkpoint interval."""
+        g = self.geom
+        mode = self.history if history is None else history
+        if mode == 'checkpoint':
+            return min(self.checkpoint_interval, g.nt)
+        if mode == 'full':
+            return g.nt
+        itemsize = torch.empty((), dtype=self.dtype).element_size()
+        full = nb * g.ns * self.H * self.W * itemsize * g.nt
+        budget = self.max_history_bytes
+        if budget is None:
+            free, _ = torch.cuda.mem_get_info(self.device)
+            budget = 0.6 * free
+        return g.nt if full <= budget else min(self.checkpoint_interval, g.nt)
+
+    # ----------------------------------------------------------- gradients
+    def _prepare(self, v_bg):
+        c2, amp = self._setup(v_bg)
+        return {'c2': c2, 'amp': amp, 'sd': self._source_derivative(v_bg),
+                'v': self._tensor(np.asarray(v_bg, np.float64)).contiguous(), 'nb': len(c2)}
+
+    def _observed(self, obs, nb):
+        g = self.geom
+        observed = self._tensor(np.asarray(obs))
+        if observed.shape != (nb, g.ns, g.nt, g.nx) or not torch.isfinite(observed).all():
+            raise ValueError(f'observations must be finite with shape {(nb, g.ns, g.nt, g.nx)}')
+        return observed.contiguous()
+
+    def _gradient_device(self, prep, observed=None, obs_repeat=1, v_mute=None, t_mute=0.12, res=None):
+        """Forward, muted residual, exact adjoint and fold on the device.
+
+        Either ``observed`` (device, (nb/obs_repeat, ns, nt, nx)) or a ready
+        data-space seed ``res`` (nb, ns, nt, nx) is given.  Returns
+        (grad (nb,ns,nz,nx), illum (nb,ns,nz,nx), sim gathers)."""
+        g = self.geom
+        nb, c2, amp, sd = prep['nb'], prep['c2'], prep['amp'], prep['sd']
+        geom = self._geom(nb)
+        H, W = self.H, self.W
+        seg = self._segment(nb)
+        p0 = self._scratch('p0', (nb, g.ns, H, W), zero=True)
+        p1 = self._scratch('p1', (nb, g.ns, H, W), zero=True)
+        sim = self._scratch('sim', (nb, g.ns, g.nt, g.nx))
+        if seg >= g.nt:
+            hist = self._scratch('hist', (nb, g.ns, g.nt, H, W))
+            self._forward(geom, 0, g.nt, c2, amp, p0, p1, gathers=sim, hist=hist)
+            ck = None
+        else:
+            nck = -(-g.nt // seg)
+            ck = self._scratch('ckpt', (nb, g.ns, nck, 2, H, W))
+            self._forward(geom, 0, g.nt, c2, amp, p0, p1, gathers=sim, ckpt=ck, every=seg)
+        if res is None:
+            res = self._scratch('res', (nb, g.ns, g.nt, g.nx))
+            f = _lib.fn('residual', self.dtype)
+            _lib.check(f(geom, _lib.ptr(sim), _lib.ptr(observed), obs_repeat, _lib.ptr(v_mute), g.dt, g.dx, t_mute,
+                         0.03, _lib.ptr(res), self._stream()), 'residual')
+        m1 = self._scratch('m1', (nb, g.ns, H, W), zero=True)
+        m2 = self._scratch('m2', (nb, g.ns, H, W), zero=True)
+        gacc = self._scratch('gacc', (nb, g.ns, H, W), zero=True)
+        illum = torch.zeros(nb, g.ns, g.nz, g.nx, dtype=self.dtype, device=self.device)
+        sg = torch.zeros(nb, g.ns, dtype=self.dtype, device=self.device)
+        if ck is None:
+            self._adjoint(geom, 0, g.nt, c2, sd, res, hist, m1, m2, gacc, illum, sg)
+        else:
+            hist = self._scratch('hist_seg', (nb, g.ns, seg, H, W))
+            for k in reversed(range(ck.shape[2])):
+                t0, t1 = k * seg, min(g.nt, (k + 1) * seg)
+                p0.copy_(ck[:, :, k, 0])
+                p1.copy_(ck[:, :, k, 1])
+                h = hist.reshape(-1)[:nb * g.ns * (t1 - t0) * H * W].view(nb, g.ns, t1 - t0, H, W)
+                self._forward(geom, t0, t1, c2, amp, p0, p1, hist=h)
+                self._adjoint(geom, t0, t1, c2, sd, res, h, m1, m2, gacc, illum, sg)
+        grad = torch.empty(nb, g.ns, g.nz, g.nx, dtype=self.dtype, device=self.device)
+        k = np.dtype(np.float32 if self.dtype == torch.float32 else np.float64).type((g.dt / g.dx) ** 2)
+        f = _lib.fn('gradient', self.dtype)
+        _lib.check(f(geom, _lib.ptr(gacc), _lib.ptr(prep['v']), _lib.ptr(sg), float(k), _lib.ptr(grad),
+                     self._stream()), 'gradient')
+        return grad, illum, sim
+
+    def _gradient(self, v_bg, obs, v_mute=None, t_mute=0.12):
+        """Per-shot misfit gradient and illumination as device tensors (gpu.py:152-188)."""
+        prep = self._prepare(v_bg)
+        observed = self._observed(obs, prep['nb'])
+        vm = None if v_mute is None else self._check_mute(v_mute)
+        grad, illum, _ = self._gradient_device(prep, observed, 1, vm, t_mute)
+        return grad, illum
+
+    @torch.no_grad()
+    def gradient(self, v_bg, obs, v_mute=None, t_mute=0.12):
+        """Per-shot derivative of .5*sum(weight*(sim-obs)**2), with fixed mute weights."""
+        grad, illum = self._gradient(v_bg, obs, v_mute, t_mute)
+        return grad.cpu().numpy(), illum.cpu().numpy()
+
+    def _image_device(self, grad, illum, eps=1e-4, laplacian=True):
+        nb = grad.shape[0]
+        img = torch.empty_like(grad)
+        f = _lib.fn('image', self.dtype)
+        e = float(np.dtype(np.float32 if self.dtype == torch.float32 else np.float64).type(eps))
+        _lib.check(f(self._geom(nb), _lib.ptr(grad), _lib.ptr(illum), e, int(bool(laplacian)), _lib.ptr(img),
+                     self._stream()), 'image')
+        return img
+
+    @torch.no_grad()
+    def images(self, v_bg, obs, v_mute=None, t_mute=0.12, eps=1e-4, laplacian=True):
+        """Return illumination-normalized negative gradients and source illumination.
+
+        The direct-wave cosine mute has a 30 ms ramp starting at offset/v_mute
+        + t_mute. Its weights are fixed during differentiation.
+        """
+        if eps <= 0:
+            raise ValueError('illumination regularization eps must be positive')
+        grad, illum = self._gradient(v_bg, obs, v_mute, t_mute)
+        img = self._image_device(grad, illum, eps, laplacian)
+        return img.cpu().numpy(), illum.cpu().numpy()
+
+    @torch.no_grad()
+    def images_device(self, prep, observed, obs_repeat=1, v_mute=None, t_mute=0.12, eps=1e-4, laplacian=True):
+        """Device-resident images: ``prep`` from ``_prepare``, ``observed`` a device tensor
+        (nb/obs_repeat, ns, nt, nx) reused by obs_repeat consecutive rows, ``v_mute`` a
+        device tensor (nb,).  Returns (img, illum) device tensors."""
+        grad, illum, _ = self._gradient_device(prep, observed, obs_repeat, v_mute, t_mute)
+        return self._image_device(grad, illum, eps, laplacian), illum
+
+    @torch.no_grad()
+    def born(self, v, dv):
+        """Tangent-linear gathers d(gathers)/dv . dv (no reference counterpart)."""
+        g = self.geom
+        c2, amp = self._setup(v)
+        vv, dvv = np.asarray(v, np.float64), np.asarray(dv, np.float64)
+        p = self.pad
+        vp = np.pad(vv, ((0, 0), (p, p), (p, p)), mode='edge')
+        dvp = np.pad(dvv, ((0, 0), (p, p), (p, p)), mode='edge')
+        dc2 = self._tensor(2 * vp * dvp * (g.dt / g.dx) ** 2)
+        damp = self._tensor(2 * vv[:, g.row, list(g.src)] * dvv[:, g.row, list(g.src)] / 2000.0 ** 2)
+        nb = len(c2)
+        shape = (nb, g.ns, self.H, self.W)
+        p0, p1, q0, q1 = (torch.zeros(shape, dtype=self.dtype, device=self.device) for _ in range(4))
+        out = torch.empty(nb, g.ns, g.nt, g.nx, dtype=self.dtype, device=self.device)
+        f = _lib.fn('born', self.dtype)
+        _lib.check(f(self._geom(nb), 0, g.nt, _lib.ptr(c2), _lib.ptr(dc2), _lib.ptr(self.A), _lib.ptr(self.B),
+                     _lib.ptr(self.w), _lib.ptr(amp), _lib.ptr(damp), _lib.ptr(p0), _lib.ptr(p1), _lib.ptr(q0),
+                     _lib.ptr(q1), _lib.ptr(out), self._stream()), 'born')
+        return out
+
+
+def calibrate(models, gathers, eps=1e-3):
+    """One effective source wavelet pooled over training models with known velocity (gpu.py:213-221)."""
+    s = Sim(RICKER)
+    sims = s.gathers(np.asarray(models, np.float64))
+    Sf = np.fft.rfft(sims, axis=2)
+    Of = np.fft.rfft(np.asarray(gathers, np.float64), axis=2)
+    num = (np.conj(Sf) * Of).sum(axis=(0, 1, 3))
+    den = (np.abs(Sf) ** 2).sum(axis=(0, 1, 3))
+    return np.fft.irfft(np.fft.rfft(RICKER) * num / (den + eps * den.max()), n=NT)

@@ -1,0 +1,206 @@
+"""GPU parity of the HIP path (through the C ABI) with the reference's golden
+vectors and the CPU oracle.
+
+Bar: gathers, illumination and the stencil operators are bit-identical to the
+reference FP32/FP64 CPU path; gradients and images (whose padding fold sums in
+a different order) are within 1e-5 relative L2 in FP32 and 1e-12 in FP64.
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import fd
+from shotsweep_amd import _lib, gpu, synth
+from shotsweep_amd.constants import Acquisition
+
+pytestmark = pytest.mark.gpu
+
+DEV = torch.device('cuda')
+F32_TOL = 1e-5
+F64_TOL = 1e-12
+SMALL = Acquisition(nz=24, nx=28, nt=260, ns=3)
+ORACLE_SMALL = fd.Geometry(nz=24, nx=28, nt=260, ns=3)
+
+
+def rel(a, b):
+    a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def case(name):
+    print(f'\n[case] {name}', flush=True)
+
+
+@pytest.fixture(autouse=True)
+def sync_after():
+    yield
+    torch.cuda.synchronize()
+
+
+@pytest.mark.parametrize('dtype', [torch.float32, torch.float64])
+def test_stencil_operators_match_oracle(dtype):
+    case(f'lap/lap_adjoint/fold {dtype}')
+    g = torch.Generator().manual_seed(8)
+    x = torch.randn(2, 3, 19, 23, generator=g, dtype=dtype)
+    np.testing.assert_array_equal(gpu.lap(x.to(DEV)).cpu().numpy(), fd.lap(x).numpy())
+    np.testing.assert_array_equal(gpu.lap_adjoint(x.to(DEV)).cpu().numpy(), fd.lap_adjoint(x).numpy())
+    y = torch.randn(2, 3, 19, 23, generator=g, dtype=dtype)
+    lhs = float((gpu.lap(x.to(DEV)) * y.to(DEV)).sum())
+    rhs = float((x.to(DEV) * gpu.lap_adjoint(y.to(DEV))).sum())
+    assert abs(lhs - rhs) <= (1e-4 if dtype == torch.float32 else 1e-10) * abs(lhs)
+    f = gpu.fold_edge_padding(x.to(DEV), 5).cpu().numpy()
+    np.testing.assert_allclose(f, fd.fold_edge_padding(x, 5, 9, 13).numpy(),
+                               rtol=1e-6 if dtype == torch.float32 else 1e-14, atol=1e-5)
+
+
+@pytest.mark.parametrize('impl,dtype,tag', [(_lib.IMPL_STEPWISE, torch.float32, 'f32'),
+                                            (_lib.IMPL_STEPWISE, torch.float64, 'f64'),
+                                            (_lib.IMPL_RESIDENT, torch.float32, 'f32')])
+def test_gathers_bit_identical_to_reference(golden, impl, dtype, tag):
+    case(f'gathers impl={impl} {tag}')
+    sim = gpu.Sim(device='cuda', dtype=dtype, impl=impl)
+    obs = sim.gathers(golden['true'])
+    np.testing.assert_array_equal(obs, golden[f'obs_{tag}'])
+
+
+@pytest.mark.parametrize('impl,dtype,tag,history', [
+    (_lib.IMPL_RESIDENT, torch.float32, 'f32', 'full'),
+    (_lib.IMPL_RESIDENT, torch.float32, 'f32', 'checkpoint'),
+    (_lib.IMPL_STEPWISE, torch.float32, 'f32', 'full'),
+    (_lib.IMPL_STEPWISE, torch.float64, 'f64', 'checkpoint'),
+])
+def test_images_match_reference(golden, impl, dtype, tag, history):
+    case(f'images impl={impl} {tag} history={history}')
+    sim = gpu.Sim(device='cuda', dtype=dtype, impl=impl, history=history)
+    obs = golden[f'obs_{tag}']
+    img, illum = sim.images(golden['backgrounds'], np.repeat(obs, 3, 0), v_mute=golden['v_mute'])
+    np.testing.assert_array_equal(illum, golden[f'illum_{tag}'])
+    tol = F32_TOL if tag == 'f32' else F64_TOL
+    e = rel(img, golden[f'img_{tag}'])
+    print(f'  image rel L2 vs reference {tag}: {e:.3e}; vs reference FP64: {rel(img, golden["img_f64"]):.3e}')
+    assert e < tol
+
+
+@pytest.mark.parametrize('impl,dtype,tag', [(_lib.IMPL_RESIDENT, torch.float32, 'f32'),
+                                            (_lib.IMPL_STEPWISE, torch.float64, 'f64')])
+def test_gradient_matches_reference(golden, impl, dtype, tag):
+    case(f'gradient impl={impl} {tag}')
+    sim = gpu.Sim(device='cuda', dtype=dtype, impl=impl)
+    grad, illum = sim.gradient(golden['backgrounds'][:1], golden[f'obs_{tag}'])
+    np.testing.assert_array_equal(illum, golden[f'grad_illum_{tag}'])
+    e = rel(grad, golden[f'grad_{tag}'])
+    print(f'  gradient rel L2 vs reference {tag}: {e:.3e}')
+    assert e < (F32_TOL if tag == 'f32' else F64_TOL)
+
+
+def test_checkpoint_recompute_equals_full_history(golden):
+    case('checkpoint vs full history bitwise')
+    obs = np.repeat(golden['obs_f32'], 3, 0)
+    full = gpu.Sim(device='cuda', history='full').images(golden['backgrounds'], obs, v_mute=golden['v_mute'])
+    ck = gpu.Sim(device='cuda', history='checkpoint', checkpoint_interval=73).images(
+        golden['backgrounds'], obs, v_mute=golden['v_mute'])
+    np.testing.assert_array_equal(full[0], ck[0])
+    np.testing.assert_array_equal(full[1], ck[1])
+
+
+@pytest.mark.parametrize('dtype', [torch.float32, torch.float64])
+def test_mute_weights_match_oracle(dtype):
+    case(f'mute weights {dtype}')
+    vm = np.array([1550.0, 2000.0, 3100.0])
+    got = gpu.Sim(device='cuda', dtype=dtype).mute_weights(vm).cpu().numpy()
+    want = fd.OracleSim(dtype=dtype).mute_weights(vm).numpy()
+    np.testing.assert_allclose(got, want, rtol=0, atol=4 * np.finfo(np.float32 if dtype == torch.float32
+                                                                      else np.float64).eps)
+
+
+def test_calibrate_matches_reference(golden):
+    case('calibrate')
+    cal_obs = 0.7 * np.roll(golden['obs_f32'].astype(np.float64), 3, axis=2)
+    np.testing.assert_array_equal(gpu.calibrate(golden['true'], cal_obs), golden['calibrated'])
+
+
+def test_mixed_models_resident_vs_oracle():
+    case('mixed models 2x3 backgrounds resident vs oracle FP32')
+    models = synth.mixed(2, seed=11)
+    sim = gpu.Sim(device='cuda')
+    osim = fd.OracleSim()
+    obs = sim.gathers(models)
+    np.testing.assert_array_equal(obs, osim.gathers(models))
+    trend = np.linspace(0, 1500, 70)
+    bgs = np.concatenate([np.stack([np.tile(np.clip(m[0].mean() + r * trend, 1500, 4500)[:, None], (1, 70))
+                                    for r in (0.6, 1.0, 1.4)]) for m in models])
+    vm = np.repeat(models[:, 0].mean(1), 3)
+    img, illum = sim.images(bgs, np.repeat(obs, 3, 0), v_mute=vm)
+    oimg, oillum = osim.images(bgs, np.repeat(obs, 3, 0), v_mute=vm)
+    np.testing.assert_array_equal(illum, oillum)
+    e = rel(img, oimg)
+    print(f'  image rel L2 vs oracle: {e:.3e}')
+    assert e < F32_TOL
+
+
+def test_rectangular_geometry_vs_oracle():
+    case('rectangular 24x28 nt=260 ns=3 stepwise+resident vs oracle')
+    rng = np.random.default_rng(2)
+    v = 1800 + 900 * rng.random((2, 24, 28))
+    want = fd.OracleSim(pad=12, geom=ORACLE_SMALL).gathers(v)
+    for impl in (_lib.IMPL_STEPWISE, _lib.IMPL_RESIDENT):
+        got = gpu.Sim(device='cuda', pad=12, geom=SMALL, impl=impl).gathers(v)
+        np.testing.assert_array_equal(got, want)
+    obs = want * 0.9
+    og, oi = fd.OracleSim(pad=12, geom=ORACLE_SMALL).gradient(v, obs, v_mute=np.array([1900.0, 2100.0]))
+    gg, gi = gpu.Sim(device='cuda', pad=12, geom=SMALL).gradient(v, obs, v_mute=np.array([1900.0, 2100.0]))
+    np.testing.assert_array_equal(gi, oi)
+    assert rel(gg, og) < F32_TOL
+
+
+def test_dot_product_fp64():
+    case('dot product <y,Fx> = <F*y,x> FP64')
+    sim = gpu.Sim(device='cuda', dtype=torch.float64, pad=12, geom=SMALL)
+    rng = np.random.default_rng(5)
+    v = 2000 + 300 * rng.random((2, SMALL.nz, SMALL.nx))
+    x = rng.standard_normal(v.shape)
+    y = torch.tensor(rng.standard_normal((2, SMALL.ns, SMALL.nt, SMALL.nx)), device=DEV)
+    Fx = sim.born(v, x)
+    grad, _, _ = sim._gradient_device(sim._prepare(v), res=y)
+    lhs = float((y * Fx).sum())
+    rhs = float((grad.sum(1).cpu().numpy() * x).sum())
+    print(f'  dot-product relative mismatch {abs(lhs - rhs) / abs(lhs):.3e}')
+    assert abs(lhs - rhs) / abs(lhs) < 1e-12
+
+
+def test_gradient_finite_differences_fp64():
+    case('finite differences FP64')
+    sim = gpu.Sim(device='cuda', dtype=torch.float64, pad=12, geom=SMALL)
+    v = np.full((1, SMALL.nz, SMALL.nx), 2000.0)
+    truth = v.copy()
+    truth[:, 8:] = 2400.0
+    obs = sim.gathers(truth)
+    grad = sim.gradient(v, obs)[0].sum(1)
+    rng = np.random.default_rng(9)
+    for k in range(4):
+        d = np.zeros_like(v)
+        d[0, rng.integers(0, SMALL.nz), rng.integers(0, SMALL.nx)] = 1
+
+        def loss(m):
+            r = sim.gathers(m) - obs
+            return 0.5 * np.sum(r * r)
+        eps = 0.1
+        num = (loss(v + eps * d) - loss(v - eps * d)) / (2 * eps)
+        assert abs(num - np.sum(grad * d)) / max(abs(num), 1.0) < 1e-4
+
+
+def test_errors_match_reference():
+    case('validation errors')
+    sim = gpu.Sim(device='cuda')
+    with pytest.raises(ValueError, match='velocities must be finite positive'):
+        sim.gathers(np.full((1, 70, 69), 2000.0))
+    with pytest.raises(ValueError, match='observations must be finite'):
+        sim.gradient(np.full((1, 70, 70), 2000.0), np.zeros((1, 5, 999, 70)))
+    with pytest.raises(ValueError, match='eps must be positive'):
+        sim.images(np.full((1, 70, 70), 2000.0), np.zeros((1, 5, 1000, 70)), eps=0)
+    with pytest.raises(ValueError, match='mute velocities'):
+        sim.mute_weights(np.array([-1.0]))
+    with pytest.raises(ValueError, match='wavelet must contain'):
+        gpu.Sim(np.zeros(1000), device='cuda')
+    with pytest.raises(RuntimeError, match='ROCm GPU'):
+        gpu.Sim(device='cpu')
