@@ -1,0 +1,144 @@
+"""Benchmark: batched forward + exact-adjoint RTM images on one MI355X.
+
+    python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--models 64] [--seed 0] [--no-cpu]
+
+Workload (one "step"): 64 seeded 70 x 70 models of mixed structure (curved
+layers, a planar fault, smoothed texture) x 3 backgrounds (surface velocity +
+0.6/1.0/1.4 x depth trend) x 5 shots = 960 shot-images; nt = 1000, dt = 1 ms,
+dx = 10 m, pad 40 (150 x 150 padded grid), 15 Hz Ricker, FP32.  One step runs
+the forward with the Laplacian history, the muted residual, the exact adjoint
+with the imaging condition, the gradient fold and the image epilogue, on
+inputs resident in HBM (observed gathers, coefficient planes, mute velocities).
+Model generation, observed-data generation and host coefficient setup are
+outside the timed region.
+
+Prints one JSON line with the throughput, the roofline of the dominant kernel
+(HIP events on the launching stream) and the host-CPU baseline (the eager
+PyTorch oracle, timed on a bounded sample in this process).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+import torch
+
+METRIC = 'RTM shot-images/s, 70x70 nt=1000 fwd+adjoint; learned vs plain refine misfit'
+PEAK_HBM_GBS = 8000.0
+BYTES_PER_CELL_UPDATE = 16          # read p(t-1), p(t), c2; write p(t+1); FP32
+
+
+def workload(n_models, seed, sim):
+    """Models, observed gathers (device), backgrounds and mute velocities."""
+    from shotsweep_amd import background as BG, synth
+    models = synth.mixed(n_models, seed=seed)
+    out, _, _ = sim.forward(models)
+    obs = out.contiguous()
+    obs_host = obs.cpu().numpy()
+    trend = BG.depth_trend(models)
+    v_s, fallbacks = [], 0
+    for k in range(n_models):
+        try:
+            v_s.append(BG.estimate_surface_velocity(obs_host[k])['velocity'])
+        except ValueError:
+            v_s.append(float(models[k, 0].mean()))
+            fallbacks += 1
+    v_s = np.asarray(v_s)
+    bgs = np.concatenate([BG.backgrounds(v, trend) for v in v_s])
+    v_mute = np.repeat(v_s, len(BG.RHOS))
+    return models, obs, obs_host, bgs, v_mute, fallbacks
+
+
+def cpu_baseline(bgs, obs_host, v_mute, n_rows):
+    """Time the oracle (eager PyTorch CPU restatement of the reference) on n_rows backgrounds."""
+    from oracle import fd
+    cores = len(os.sched_getaffinity(0))
+    torch.set_num_threads(cores)
+    osim = fd.OracleSim()
+    obs_rep = np.repeat(obs_host, 3, 0)[:n_rows]
+    t0 = time.perf_counter()
+    img, illum = osim.images(bgs[:n_rows], obs_rep, v_mute=v_mute[:n_rows])
+    dt = time.perf_counter() - t0
+    shot_images = n_rows * 5
+    return {'value': shot_images / dt, 'unit': 'shot-images/s', 'cores': cores, 'kind': 'port',
+            'sample': f'{n_rows} background rows ({shot_images} shot-images) of the same workload, oracle FP32 '
+                      f'eager PyTorch on {cores} threads, one run of {dt:.1f} s'}, img, illum
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=3)
+    ap.add_argument('--warmup', type=int, default=1)
+    ap.add_argument('--models', type=int, default=64)
+    ap.add_argument('--seed', type=int, default=0)
+    ap.add_argument('--history', default='auto', choices=('auto', 'full', 'checkpoint'))
+    ap.add_argument('--no-cpu', action='store_true')
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) != 1:
+        print('bench.py measures one GPU: the path shards by model with no exchange in the time loop; '
+              'multi-GPU runs are not part of this build', file=sys.stderr)
+        sys.exit(2)
+    from shotsweep_amd import gpu
+
+    dev = torch.device('cuda')
+    sim = gpu.Sim(device=dev, history=args.history)
+    models, obs, obs_host, bgs, v_mute, fallbacks = workload(args.models, args.seed, sim)
+    prep = sim._prepare(bgs)
+    vm = sim._tensor(v_mute).contiguous()
+    nb = prep['nb']
+    shot_images = nb * sim.geom.ns
+    print(f'[bench] {args.models} models -> {nb} backgrounds -> {shot_images} shot-images; '
+          f'history segment {sim._segment(nb)}; surface-velocity fallbacks {fallbacks}', file=sys.stderr, flush=True)
+
+    for _ in range(args.warmup):
+        sim.images_device(prep, obs, 3, vm)
+    torch.cuda.synchronize()
+    sim.events = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        img, illum = sim.images_device(prep, obs, 3, vm)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    events, sim.events = sim.events, None
+    ms_per_step = wall / args.steps * 1e3
+    value = shot_images / (wall / args.steps)
+
+    per_kernel = {}
+    for name, a, b in events:
+        per_kernel.setdefault(name, []).append(a.elapsed_time(b))
+    avg = {k: float(np.mean(v)) for k, v in per_kernel.items()}
+    dominant = max(avg, key=avg.get)
+    H, W, nt = sim.H, sim.W, sim.geom.nt
+    alg_bytes = BYTES_PER_CELL_UPDATE * H * W * nt * shot_images       # one cell-update per cell-step per launch
+    achieved = alg_bytes / (avg[dominant] * 1e-3) / 1e9
+    print(f'[bench] {ms_per_step:.1f} ms/step, kernels {avg}', file=sys.stderr, flush=True)
+
+    result = {
+        'metric': METRIC, 'value': value, 'unit': 'shot-images/s', 'n_gpus': 1, 'steps': args.steps,
+        'warmup': args.warmup, 'ms_per_step': ms_per_step, 'higher_is_better': True, 'scaling': 'weak',
+        'vs_baseline': None, 'dtype': 'f32', 'data': 'synthetic (seeded mixed-structure models)',
+        'config': {'workload': 'batched RTM: 64 mixed 70x70 models x 3 backgrounds x 5 shots = 960 shot-images, '
+                               'nt=1000, dt=1 ms, dx=10 m, pad 40 (150x150), 15 Hz Ricker',
+                   'models': args.models, 'shot_images': shot_images, 'seed': args.seed,
+                   'history_segment': sim._segment(nb)},
+        'roofline': {'bound': 'hbm', 'kernel': dominant, 'achieved': achieved, 'peak': PEAK_HBM_GBS, 'unit': 'GB/s',
+                     'frac': achieved / PEAK_HBM_GBS, 'traffic': None,
+                     'kernel_ms': avg, 'algorithmic_bytes_per_launch': alg_bytes},
+    }
+    if not args.no_cpu:
+        rows = 3                               # one model: 3 backgrounds x 5 shots
+        base, cimg, cillum = cpu_baseline(bgs, obs_host, v_mute, rows)
+        gimg = img[:rows].cpu().numpy()
+        err = float(np.linalg.norm(gimg - cimg) / np.linalg.norm(cimg))
+        result['cpu_baseline'] = base
+        result['speedup_vs_cpu'] = value / base['value']
+        result['image_rel_l2_vs_cpu_oracle'] = err
+        result['illum_bitwise_equal_cpu_oracle'] = bool(np.array_equal(illum[:rows].cpu().numpy(), cillum))
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == '__main__':
+    main()

@@ -136,8 +136,8 @@ class Sim:
         self.w = self._tensor(self.wav * 1e3)
         self.si = torch.arange(g.ns, device=self.device)
         self.sx = torch.tensor(list(g.src), device=self.device) + pad
-        self._geom_c = None
         self._ws = {}
+        self.events = None   # list -> (name, start, end) HIP events around each time-loop launch
 
     # ---------------------------------------------------------------- setup
     def _damping(self, R):
@@ -171,15 +171,15 @@ class Sim:
         p = self.pad
         vp = np.pad(v, ((0, 0), (p, p), (p, p)), mode='edge')
         c2 = self._tensor(vp ** 2 * (g.dt / g.dx) ** 2)[:, None]
-        amp = self._tensor((v[:, g.row, list(g.src)] / 2000.0) ** 2)
+        amp = self._tensor(np.ascontiguousarray((v[:, g.row, list(g.src)] / 2000.0) ** 2))
         return c2, amp
 
     def _source_derivative(self, v):
         """2 v[row, src] / 2000^2 in the working dtype (gpu.py:168), formed on the host
         with the reference's CPU division."""
         g = self.geom
-        sd = 2 * self._host(np.asarray(v, np.float64)[:, g.row, list(g.src)]) / 2000.0 ** 2
-        return sd.to(self.device)
+        sd = 2 * self._host(np.ascontiguousarray(np.asarray(v, np.float64)[:, g.row, list(g.src)])) / 2000.0 ** 2
+        return sd.contiguous().to(self.device)
 
     def _scratch(self, name, shape, zero=False):
         t = self._ws.get(name)
@@ -201,17 +201,31 @@ class Sim:
     def _stream(self):
         return _lib.stream_handle(self.device)
 
+    def _timed(self, name):
+        if self.events is None:
+            return None
+        start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        start.record()
+        self.events.append((name, start, end))
+        return end
+
     def _forward(self, geom, t0, t1, c2, amp, p0, p1, gathers=None, hist=None, ckpt=None, every=1, impl=None):
         f = _lib.fn('forward', self.dtype)
+        end = self._timed('forward')
         _lib.check(f(geom, t0, t1, _lib.ptr(c2), _lib.ptr(self.A), _lib.ptr(self.B), _lib.ptr(self.w),
                      _lib.ptr(amp), _lib.ptr(p0), _lib.ptr(p1), _lib.ptr(gathers), _lib.ptr(hist), _lib.ptr(ckpt),
                      every, self.impl if impl is None else impl, self._stream()), 'forward')
+        if end is not None:
+            end.record()
 
     def _adjoint(self, geom, t0, t1, c2, sd, res, hist, m1, m2, gacc, illum, sg, impl=None):
         f = _lib.fn('adjoint', self.dtype)
+        end = self._timed('adjoint')
         _lib.check(f(geom, t0, t1, _lib.ptr(c2), _lib.ptr(self.A), _lib.ptr(self.B), _lib.ptr(self.w),
                      _lib.ptr(sd), _lib.ptr(res), _lib.ptr(hist), _lib.ptr(m1), _lib.ptr(m2), _lib.ptr(gacc),
                      _lib.ptr(illum), _lib.ptr(sg), self.impl if impl is None else impl, self._stream()), 'adjoint')
+        if end is not None:
+            end.record()
 
     def _step(self, p0, p1, c2, amp, it):
         """One damped leapfrog step; returns (p2, L) (gpu.py:111-115)."""
@@ -410,7 +424,8 @@ class Sim:
         vp = np.pad(vv, ((0, 0), (p, p), (p, p)), mode='edge')
         dvp = np.pad(dvv, ((0, 0), (p, p), (p, p)), mode='edge')
         dc2 = self._tensor(2 * vp * dvp * (g.dt / g.dx) ** 2)
-        damp = self._tensor(2 * vv[:, g.row, list(g.src)] * dvv[:, g.row, list(g.src)] / 2000.0 ** 2)
+        damp = self._tensor(np.ascontiguousarray(2 * vv[:, g.row, list(g.src)] * dvv[:, g.row, list(g.src)]
+                                                 / 2000.0 ** 2))
         nb = len(c2)
         shape = (nb, g.ns, self.H, self.W)
         p0, p1, q0, q1 = (torch.zeros(shape, dtype=self.dtype, device=self.device) for _ in range(4))
