@@ -11,7 +11,7 @@
 #include <stdint.h>
 #include <string>
 
-#include "../../include/shotsweep.h"
+#include "shotsweep.h"
 
 namespace ss {
 
@@ -77,16 +77,18 @@ __host__ __device__ __forceinline__ T lapt_sum(T qc, T qzp2, T qzp1, T qzm1, T q
 }
 
 // Correctly rounded s/12: the reciprocal product plus one FMA correction
-// equals IEEE division for every finite s with |s| >= 2^-119 (checked on the
-// host over every mantissa and the whole small-magnitude range by
-// tests/native/div12_check.cpp); smaller or non-finite s take IEEE division.
+// equals IEEE division for every finite s with |s| >= 2^-119, and for s = +-0
+// once the sign is restored (checked on the host over every mantissa and the
+// whole small-magnitude range by tests/native/div12_check.cpp).  Tiny nonzero
+// or non-finite s take IEEE division; exact zeros, common ahead of the
+// wavefront, stay on the fast path.
 __host__ __device__ __forceinline__ float div12(float s) {
   const float R = 0x1.555556p-4f;  // RN(1/12)
   const float a = __builtin_fabsf(s);
-  if (!(a >= 0x1p-119f && a <= 0x1.fffffep+127f)) return s / 12.0f;
+  if (!((a >= 0x1p-119f && a <= 0x1.fffffep+127f) || a == 0.0f)) return s / 12.0f;
   const float q0 = s * R;
   const float r = __builtin_fmaf(-q0, 12.0f, s);
-  return __builtin_fmaf(r, R, q0);
+  return __builtin_copysignf(__builtin_fmaf(r, R, q0), s);
 }
 
 // Damped leapfrog update (gpu.py:113): ((2 p1 - B p0) + c2 L) A.

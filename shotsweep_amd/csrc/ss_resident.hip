@@ -10,16 +10,20 @@
 // HBM for the adjoint (gpu.py:172-175).
 //
 // Adjoint (gpu.py:177-184): LDS holds q = (c2 A m1)/12 (zero on the two-cell
-// border), the illumination accumulator of the model cells and the sponge
-// table; each thread keeps m1, m2, the imaging accumulator and c2*A of its
-// cells in registers and streams the forward Laplacian from HBM.
+// border), the illumination accumulator of the model cells, the sponge table
+// and row/column descriptors; each thread keeps m1, m2 and the imaging
+// accumulator of its cells in registers (512 threads, 44 cells each), streams
+// the forward Laplacian from HBM a few cells ahead of use, and re-reads c2
+// from the L2-resident c2 plane when it publishes q.
 //
 // Cell ownership: cell i of the H*W plane belongs to thread i % 1024 as its
 // slot j = i / 1024, so every LDS and HBM access of a wavefront touches 64
 // consecutive cells.  Sponge coefficients A, B (gpu.py:85-90) depend only on
 // the distance (dz, dx) into the sponge, so they come from an LDS table of
-// (pad+1)^2 entries; each cell carries a 16-bit descriptor (table index or,
-// for model cells, the illumination index, plus two flags), two per register.
+// (pad+1)^2 entries.  In the forward each cell carries a 16-bit descriptor
+// (table index or, for model cells, the illumination index, plus two flags),
+// two per register; the adjoint, short of registers, derives the same from
+// per-row and per-column LDS tables.
 //
 // The 150 x 150 plane (70 x 70 model, pad 40) is compiled with its shape as
 // constants, so stencil offsets fold into instruction immediates; other shapes
@@ -32,8 +36,22 @@ namespace ss {
 
 namespace {
 
-constexpr int kThreads = 1024;
-constexpr int kMaxSlots = 24;
+#ifndef SS_ADJ_THREADS
+#define SS_ADJ_THREADS 512
+#endif
+#ifndef SS_GROUP
+#define SS_GROUP 2
+#endif
+#ifndef SS_PREFETCH
+#define SS_PREFETCH 4
+#endif
+constexpr int kFwdThreads = 1024;            // forward: p(t-1), c2 + descriptors = 2.5 registers per cell
+constexpr int kAdjThreads = SS_ADJ_THREADS;  // adjoint: m1, m2, imaging sum = 3 registers per cell
+constexpr int kFwdMaxSlots = 24;
+constexpr int kAdjMaxSlots = 48;
+constexpr int kPrefetch = SS_PREFETCH;       // adjoint: Laplacian loads issued this many cells ahead
+// slots of the 150 x 150 plane, rounded up to a multiple of the prefetch distance
+constexpr int kAdj150 = ((150 * 150 + kAdjThreads - 1) / kAdjThreads + kPrefetch - 1) / kPrefetch * kPrefetch;
 constexpr uint32_t kBufWord3 = 0x00020000;  // raw buffer descriptor word 3 for the gfx9 family
 
 // 16-bit cell descriptor
@@ -41,12 +59,21 @@ constexpr uint32_t kIdxMask = 0x1fff;  // sponge table index, or illumination in
 constexpr uint32_t kInterior = 1u << 13;
 constexpr uint32_t kModel = 1u << 14;
 
-// Keep the scheduler from hoisting the LDS taps of many cells at once (which
-// exhausts the registers that hold per-cell state): cells are scheduled in
-// groups of kGroup.
-constexpr int kGroup = 4;
-__device__ __forceinline__ void fence_cells(int j) {
-  if ((j + 1) % kGroup == 0) __builtin_amdgcn_sched_barrier(0);
+// Cells are processed in groups of kGroup.  Before each group an empty asm
+// "redefines" the base offset that its LDS and buffer addresses are formed
+// from, and its descriptors, so that no work of a later group can be hoisted
+// above it: otherwise the compiler interleaves all cells of a step and runs
+// out of registers for the per-cell state.
+constexpr int kGroup = SS_GROUP;
+__device__ __forceinline__ void group_fence(int j, int &off) {
+  if (j % kGroup == 0) asm volatile("" : "+v"(off));
+}
+__device__ __forceinline__ void group_fence(int j, int &off, uint32_t *pk) {
+  if (j % kGroup == 0) {
+    asm volatile("" : "+v"(off));
+#pragma unroll
+    for (int k = j; k < j + kGroup; k += 2) asm volatile("" : "+v"(pk[k >> 1]));
+  }
 }
 
 __device__ __forceinline__ int sponge_dist(int i, int n, int pad) {
@@ -56,15 +83,24 @@ __device__ __forceinline__ int sponge_dist(int i, int n, int pad) {
   return d > 0 ? d : 0;
 }
 
-__device__ __forceinline__ uint32_t cell_desc(int i, int H, int W, const Geom &g) {
+__device__ __forceinline__ uint32_t cell_desc(int i, int H, int W, int nz, int nx, int pad) {
   const int z = i / W, x = i - z * W;
   uint32_t v;
-  if (z >= g.pad && z < g.pad + g.nz && x >= g.pad && x < g.pad + g.nx)
-    v = kModel | (uint32_t)((z - g.pad) * g.nx + (x - g.pad));
+  if (z >= pad && z < pad + nz && x >= pad && x < pad + nx)
+    v = kModel | (uint32_t)((z - pad) * nx + (x - pad));
   else
-    v = (uint32_t)(sponge_dist(z, H, g.pad) * (g.pad + 1) + sponge_dist(x, W, g.pad));
+    v = (uint32_t)(sponge_dist(z, H, pad) * (pad + 1) + sponge_dist(x, W, pad));
   if (z >= 2 && z < H - 2 && x >= 2 && x < W - 2) v |= kInterior;
   return v;
+}
+
+// Per-cell descriptors are loop-invariant; left visible, the compiler hoists
+// every value derived from them (table addresses, flag masks) out of the time
+// loop and spills.  An empty asm "redefines" them once per step instead.
+template <int N>
+__device__ __forceinline__ void opaque(uint32_t (&pk)[N]) {
+#pragma unroll
+  for (int j = 0; j < N; ++j) asm volatile("" : "+v"(pk[j]));
 }
 
 __device__ __forceinline__ uint32_t desc_of(const uint32_t *pk, int j) {
@@ -80,12 +116,27 @@ __device__ __forceinline__ float2 sponge(const float2 *tab, uint32_t d) {
 // planes at the top-left sponge cell (pad-dz, pad-dx).
 __device__ void fill_table(float2 *tab, const float *A, const float *B, int W, int pad, int mode) {
   const int n = (pad + 1) * (pad + 1);
-  for (int k = threadIdx.x; k < n; k += kThreads) {
+  for (int k = threadIdx.x; k < n; k += blockDim.x) {
     const int dz = k / (pad + 1), dx = k - dz * (pad + 1);
     const int c = (pad - dz) * W + (pad - dx);
     const float a = A[c], b = B[c];
     tab[k] = make_float2(a, mode ? a * b : b);
   }
+}
+
+// Workgroup -> propagation.  Workgroups are dealt round-robin to the 8 XCDs
+// (blockIdx % 8), so XCD x receives blocks x, x+8, ...; mapping those to the
+// contiguous propagation range [x*per, (x+1)*per) keeps the shots of one
+// velocity row (which share their c2 plane) inside one XCD's L2.
+constexpr int kXcds = 8;
+__host__ __device__ __forceinline__ int64_t xcd_blocks(int64_t nprop) {
+  return (nprop + kXcds - 1) / kXcds * kXcds;
+}
+__device__ __forceinline__ int64_t xcd_prop(const Geom &g) {
+  const int64_t nprop = (int64_t)g.nb * g.ns;
+  const int64_t per = xcd_blocks(nprop) / kXcds;
+  const int64_t p = (int64_t)(blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
+  return p < nprop ? p : -1;
 }
 
 // LDS layout: [guard | plane H*W | guard | (align) | sponge table | illumination]
@@ -98,7 +149,20 @@ size_t fwd_lds_bytes(const Geom &g) {
   return (size_t)table_offset(g.H, g.W) * sizeof(float) + (size_t)(g.pad + 1) * (g.pad + 1) * sizeof(float2);
 }
 
-size_t adj_lds_bytes(const Geom &g) { return fwd_lds_bytes(g) + (size_t)g.nz * g.nx * sizeof(float); }
+size_t adj_lds_bytes(const Geom &g) {
+  return fwd_lds_bytes(g) + (size_t)g.nz * g.nx * sizeof(float) + (size_t)(g.H + g.W) * sizeof(uint32_t);
+}
+
+// Row / column descriptor of the adjoint: bits 0-7 distance into the sponge,
+// bit 8 inside the Laplacian interior [2, n-3], bit 9 inside the model
+// [pad, pad+n_model), bits 16-31 offset from the model's first row/column.
+constexpr uint32_t kLineInterior = 1u << 8, kLineModel = 1u << 9;
+__device__ __forceinline__ uint32_t line_desc(int i, int n, int nmodel, int pad) {
+  uint32_t v = (uint32_t)sponge_dist(i, n, pad);
+  if (i >= 2 && i < n - 2) v |= kLineInterior;
+  if (i >= pad && i < pad + nmodel) v |= kLineModel | ((uint32_t)(i - pad) << 16);
+  return v;
+}
 
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const float *base, int64_t nfloats) {
   const int64_t bytes = nfloats * 4;
@@ -114,8 +178,9 @@ __device__ __forceinline__ float buf_load(__amdgpu_buffer_rsrc_t r, int voff, in
   return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
 }
 
-template <int K, int HC, int WC>
-__global__ __launch_bounds__(kThreads) void fwd_resident_kernel(Geom g, int t0, int t1, const float *__restrict__ c2,
+// NZ, NX, PAD > 0 and ROW >= 0 fix the geometry at compile time (0 / -1: runtime).
+template <int K, int NZ, int NX, int PAD, int ROW>
+__global__ __launch_bounds__(kFwdThreads) void fwd_resident_kernel(Geom g, int t0, int t1, const float *__restrict__ c2,
                                                                const float *__restrict__ A,
                                                                const float *__restrict__ B,
                                                                const float *__restrict__ w,
@@ -124,20 +189,22 @@ __global__ __launch_bounds__(kThreads) void fwd_resident_kernel(Geom g, int t0, 
                                                                float *__restrict__ hist, float *__restrict__ ckpt,
                                                                int ckpt_every) {
   extern __shared__ float lds[];
-  const int H = HC ? HC : g.H, W = WC ? WC : g.W;
+  const int nz = NZ ? NZ : g.nz, nx = NX ? NX : g.nx, pad = PAD ? PAD : g.pad, row = ROW >= 0 ? ROW : g.row;
+  const int H = nz + 2 * pad, W = nx + 2 * pad;
   const int HW = H * W;
   const int guard = 2 * W + 2;
   float *P = lds + guard;
   float2 *tab = reinterpret_cast<float2 *>(lds + table_offset(H, W));
-  const int64_t prop = blockIdx.x;
+  const int64_t prop = xcd_prop(g);
+  if (prop < 0) return;  // padding workgroup (whole workgroup exits before any barrier)
   const int b = (int)(prop / g.ns), s = (int)(prop - (int64_t)b * g.ns);
   const int t = threadIdx.x;
 
-  for (int k = t; k < guard; k += kThreads) {
+  for (int k = t; k < guard; k += kFwdThreads) {
     lds[k] = 0.0f;
     P[HW + k] = 0.0f;
   }
-  fill_table(tab, A, B, W, g.pad, 0);
+  fill_table(tab, A, B, W, pad, 0);
 
   float pn[K], cc[K];
   uint32_t pk[(K + 1) / 2];
@@ -148,66 +215,70 @@ __global__ __launch_bounds__(kThreads) void fwd_resident_kernel(Geom g, int t0, 
   for (int j = 0; j < (K + 1) / 2; ++j) pk[j] = 0;
 #pragma unroll
   for (int j = 0; j < K; ++j) {
-    const int i = t + j * kThreads;
+    const int i = t + j * kFwdThreads;
     pn[j] = 0.0f;
     cc[j] = 0.0f;
     if (i < HW) {
       pn[j] = p0p[i];
       P[i] = p1p[i];
       cc[j] = c2b[i];
-      pk[j >> 1] |= cell_desc(i, H, W, g) << (16 * (j & 1));
+      pk[j >> 1] |= cell_desc(i, H, W, nz, nx, pad) << (16 * (j & 1));
     }
   }
-  const int rec0 = (g.pad + g.row) * W + g.pad;
+  const int rec0 = (pad + row) * W + pad;
   const int srci = rec0 + g.src[s];
-  const bool src_owner = (srci % kThreads) == t;
+  const bool src_owner = (srci % kFwdThreads) == t;
   const float ampv = amp[prop];
   const int nslots = t1 - t0;
   const int nck = ckpt ? (nslots + ckpt_every - 1) / ckpt_every : 0;
-  const bool rec_thread = gathers && t < g.nx;
+  const bool rec_thread = gathers && t < nx;
   // Per-propagation buffer resources: every global access inside the time
   // loop is a buffer instruction with a per-thread byte offset and a uniform
   // (scalar) offset, so no per-cell 64-bit address stays live across steps.
   const __amdgpu_buffer_rsrc_t hrs = make_rsrc(hist ? hist + prop * nslots * HW : p0g, (int64_t)nslots * HW);
   const __amdgpu_buffer_rsrc_t grs =
-      make_rsrc(gathers ? gathers + prop * g.nt * g.nx : p0g, (int64_t)g.nt * g.nx);
+      make_rsrc(gathers ? gathers + prop * g.nt * nx : p0g, (int64_t)g.nt * nx);
   const __amdgpu_buffer_rsrc_t crs = make_rsrc(ckpt ? ckpt + prop * nck * 2 * HW : p0g, nck * 2 * HW);
   __syncthreads();
 
   for (int it = t0; it < t1; ++it) {
     const int slot = it - t0;
+    opaque(pk);
     if (rec_thread && it > t0)  // p(it) after injection is the sample of step it-1
-      buf_store(P[rec0 + t], grs, t * 4, (it - 1) * g.nx * 4);
+      buf_store(P[rec0 + t], grs, t * 4, (it - 1) * nx * 4);
     if (ckpt && slot % ckpt_every == 0) {
       const int base = (slot / ckpt_every) * 2 * HW;
 #pragma unroll
       for (int j = 0; j < K; ++j) {
-        const int i = t + j * kThreads;
+        const int i = t + j * kFwdThreads;
         if (i < HW) {
-          buf_store(pn[j], crs, t * 4, (base + j * kThreads) * 4);
-          buf_store(P[i], crs, t * 4, (base + HW + j * kThreads) * 4);
+          buf_store(pn[j], crs, t * 4, (base + j * kFwdThreads) * 4);
+          buf_store(P[i], crs, t * 4, (base + HW + j * kFwdThreads) * 4);
         }
       }
     }
+    int off = t;
 #pragma unroll
     for (int j = 0; j < K; ++j) {
-      const int i = t + j * kThreads;
-      if (i < HW) {
+      group_fence(j, off, pk);
+      if (t + j * kFwdThreads < HW) {
+        const int i = off + j * kFwdThreads;
         const uint32_t d = desc_of(pk, j);
         const float c = P[i];
         const float sum = lap_sum(P[i - 2 * W], P[i - W], c, P[i + W], P[i + 2 * W], P[i - 2], P[i - 1], P[i + 1],
                                   P[i + 2]);
-        const float L = (d & kInterior) ? div12(sum) : 0.0f;
+        const float Lq = div12(sum);
+        const float L = (d & kInterior) ? Lq : 0.0f;
         const float2 ab = sponge(tab, d);
         pn[j] = leapfrog(c, pn[j], ab.y, cc[j], L, ab.x);
-        if (hist) buf_store(L, hrs, t * 4, (slot * HW + j * kThreads) * 4);
+        if (hist) buf_store(L, hrs, off * 4, (slot * HW + j * kFwdThreads) * 4);
+        asm volatile("" : "+v"(pn[j]));  // finish this cell before the next group starts
       }
-      fence_cells(j);
     }
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < K; ++j) {
-      const int i = t + j * kThreads;
+      const int i = t + j * kFwdThreads;
       if (i < HW) {
         const float c = P[i];
         P[i] = pn[j];
@@ -217,12 +288,12 @@ __global__ __launch_bounds__(kThreads) void fwd_resident_kernel(Geom g, int t0, 
     if (src_owner) P[srci] = P[srci] + w[it] * ampv;  // gpu.py:114, added after damping
     __syncthreads();
   }
-  if (rec_thread) buf_store(P[rec0 + t], grs, t * 4, (t1 - 1) * g.nx * 4);
+  if (rec_thread) buf_store(P[rec0 + t], grs, t * 4, (t1 - 1) * nx * 4);
   float *p0o = p0g + prop * HW;
   float *p1o = p1g + prop * HW;
 #pragma unroll
   for (int j = 0; j < K; ++j) {
-    const int i = t + j * kThreads;
+    const int i = t + j * kFwdThreads;
     if (i < HW) {
       p0o[i] = pn[j];
       p1o[i] = P[i];
@@ -230,8 +301,10 @@ __global__ __launch_bounds__(kThreads) void fwd_resident_kernel(Geom g, int t0, 
   }
 }
 
-template <int K, int HC, int WC>
-__global__ __launch_bounds__(kThreads) void adj_resident_kernel(Geom g, int t0, int t1, const float *__restrict__ c2,
+
+// NZ, NX, PAD > 0 and ROW >= 0 fix the geometry at compile time (0 / -1: runtime).
+template <int K, int NZ, int NX, int PAD, int ROW>
+__global__ __launch_bounds__(kAdjThreads) void adj_resident_kernel(Geom g, int t0, int t1, const float *__restrict__ c2,
                                                                const float *__restrict__ A,
                                                                const float *__restrict__ B,
                                                                const float *__restrict__ w,
@@ -242,106 +315,146 @@ __global__ __launch_bounds__(kThreads) void adj_resident_kernel(Geom g, int t0, 
                                                                float *__restrict__ illumg,
                                                                float *__restrict__ src_grad) {
   extern __shared__ float lds[];
-  const int H = HC ? HC : g.H, W = WC ? WC : g.W;
+  const int nz = NZ ? NZ : g.nz, nx = NX ? NX : g.nx, pad = PAD ? PAD : g.pad, row = ROW >= 0 ? ROW : g.row;
+  const int H = nz + 2 * pad, W = nx + 2 * pad;
   const int HW = H * W;
   const int guard = 2 * W + 2;
   float *Q = lds + guard;
   float2 *tab = reinterpret_cast<float2 *>(lds + table_offset(H, W));
-  float *il = reinterpret_cast<float *>(tab + (g.pad + 1) * (g.pad + 1));
-  const int64_t prop = blockIdx.x;
+  float *il = reinterpret_cast<float *>(tab + (pad + 1) * (pad + 1));
+  uint32_t *zinfo = reinterpret_cast<uint32_t *>(il + nz * nx);
+  uint32_t *xinfo = zinfo + H;
+  const int64_t prop = xcd_prop(g);
+  if (prop < 0) return;
   const int b = (int)(prop / g.ns), s = (int)(prop - (int64_t)b * g.ns);
   const int t = threadIdx.x;
-  const int nzx = g.nz * g.nx;
-
-  for (int k = t; k < guard; k += kThreads) {
+  const int nzx = nz * nx;
+  for (int k = t; k < guard; k += kAdjThreads) {
     lds[k] = 0.0f;
     Q[HW + k] = 0.0f;
   }
-  fill_table(tab, A, B, W, g.pad, 1);
-  for (int k = t; k < nzx; k += kThreads) il[k] = illumg[prop * nzx + k];
-  __syncthreads();  // table ready
-
-  float m1[K], m2[K], ga[K], ca[K];
-  uint32_t pk[(K + 1) / 2];
+  fill_table(tab, A, B, W, pad, 1);
+  for (int k = t; k < nzx; k += kAdjThreads) il[k] = illumg[prop * nzx + k];
+  for (int k = t; k < H; k += kAdjThreads) zinfo[k] = line_desc(k, H, nz, pad);
+  for (int k = t; k < W; k += kAdjThreads) xinfo[k] = line_desc(k, W, nx, pad);
+  __syncthreads();
+  struct Desc { uint32_t tab; bool interior, model; int illum; };
+  // i / W: a constant divisor folds to a multiply; a runtime W uses
+  // ceil(2^32 / W), exact for i * W < 2^32 (planes here hold < 2^16 cells).
+  const uint32_t magic_w = 0xffffffffu / (uint32_t)W + 1u;
+  auto desc = [&](int i) __attribute__((always_inline)) {
+    const int z = NX ? i / W : (int)__umulhi((uint32_t)i, magic_w);
+    const int x = i - z * W;
+    const uint32_t zi = zinfo[z], xi = xinfo[x];
+    Desc d;
+    d.model = (zi & xi & kLineModel) != 0;
+    d.interior = (zi & xi & kLineInterior) != 0;
+    d.tab = d.model ? 0u : (zi & 0xffu) * (uint32_t)(pad + 1) + (xi & 0xffu);
+    d.illum = (int)(zi >> 16) * nx + (int)(xi >> 16);
+    return d;
+  };
+  const __amdgpu_buffer_rsrc_t srs1 = make_rsrc(m1g + prop * HW, HW);
+  const __amdgpu_buffer_rsrc_t srs2 = make_rsrc(m2g + prop * HW, HW);
+  const __amdgpu_buffer_rsrc_t srsg = make_rsrc(gaccg + prop * HW, HW);
   const float *c2b = c2 + (int64_t)b * HW;
-#pragma unroll
-  for (int j = 0; j < (K + 1) / 2; ++j) pk[j] = 0;
+  const __amdgpu_buffer_rsrc_t crs = make_rsrc(c2b, (int64_t)HW);
+  float m1[K], m2[K], ga[K];
 #pragma unroll
   for (int j = 0; j < K; ++j) {
-    const int i = t + j * kThreads;
-    ca[j] = m1[j] = m2[j] = ga[j] = 0.0f;
-    if (i < HW) {
-      const uint32_t d = cell_desc(i, H, W, g);
-      pk[j >> 1] |= d << (16 * (j & 1));
-      ca[j] = c2b[i] * sponge(tab, d).x;  // c2 * A (gpu.py:179)
-      m1[j] = m1g[prop * HW + i];
-      m2[j] = m2g[prop * HW + i];
-      ga[j] = gaccg[prop * HW + i];
-      Q[i] = (d & kInterior) ? div12(ca[j] * m1[j]) : 0.0f;
+    m1[j] = m2[j] = ga[j] = 0.0f;
+    if (t + j * kAdjThreads < HW) {
+      m1[j] = buf_load(srs1, t * 4, j * kAdjThreads * 4);
+      m2[j] = buf_load(srs2, t * 4, j * kAdjThreads * 4);
+      ga[j] = buf_load(srsg, t * 4, j * kAdjThreads * 4);
     }
   }
-  const int rec0 = (g.pad + g.row) * W + g.pad, rec1 = rec0 + g.nx;
+#pragma unroll
+  for (int j = 0; j < K; ++j) {
+    const int i = t + j * kAdjThreads;
+    if (i < HW) {
+      const Desc d = desc(i);
+      Q[i] = d.interior ? div12((buf_load(crs, t * 4, j * kAdjThreads * 4) * tab[d.tab].x) * m1[j]) : 0.0f;
+    }
+  }
+  const int rec0 = (pad + row) * W + pad, rec1 = rec0 + nx;
   const int srci = rec0 + g.src[s];
   const float sdv = sd[prop];
-  const bool own_src = (srci % kThreads) == t;
+  const bool own_src = (srci % kAdjThreads) == t;
   float sg = own_src ? src_grad[prop] : 0.0f;
   const int nslots = t1 - t0;
-  const __amdgpu_buffer_rsrc_t rrs = make_rsrc(res + prop * g.nt * g.nx, (int64_t)g.nt * g.nx);
+  const __amdgpu_buffer_rsrc_t rrs = make_rsrc(res + prop * g.nt * nx, (int64_t)g.nt * nx);
   const __amdgpu_buffer_rsrc_t hrs = make_rsrc(hist + prop * nslots * HW, (int64_t)nslots * HW);
   __syncthreads();
-
+  constexpr int D = K < kPrefetch ? K : kPrefetch;
+  static_assert(K % D == 0, "slot count must be a multiple of the prefetch distance");
+  float Lb[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d)
+    Lb[d] = (t + d * kAdjThreads < HW) ? buf_load(hrs, t * 4, ((t1 - 1 - t0) * HW + d * kAdjThreads) * 4) : 0.0f;
   for (int it = t1 - 1; it >= t0; --it) {
     const int slot = it - t0;
     const float wt = w[it];
+    int off = t;
 #pragma unroll
     for (int j = 0; j < K; ++j) {
-      const int i = t + j * kThreads;
-      if (i < HW) {
-        const float L = buf_load(hrs, t * 4, (slot * HW + j * kThreads) * 4);
-        const uint32_t d = desc_of(pk, j);
+      group_fence(j, off);
+      const float L = Lb[j % D];
+      {
+        const int jn = j + D;
+        if (jn < K) {
+          if (t + jn * kAdjThreads < HW) Lb[j % D] = buf_load(hrs, off * 4, (slot * HW + jn * kAdjThreads) * 4);
+        } else if (it > t0 && t + (jn - K) * kAdjThreads < HW) {
+          Lb[j % D] = buf_load(hrs, off * 4, ((slot - 1) * HW + (jn - K) * kAdjThreads) * 4);
+        }
+      }
+      if (t + j * kAdjThreads < HW) {
+        const int i = off + j * kAdjThreads;
+        const Desc d = desc(i);
         const float lt = lapt_sum(Q[i], Q[i + 2 * W], Q[i + W], Q[i - W], Q[i - 2 * W], Q[i + 2], Q[i + 1],
                                   Q[i - 1], Q[i - 2]);
-        const float2 aab = sponge(tab, d);
-        // gpu.py:179: (2A) m1 + lapT - (AB) m2, AB = A*B rounded (table mode 1)
+        const float2 aab = tab[d.tab];
         float mu = (2.0f * aab.x) * m1[j];
         mu = mu + lt;
         mu = mu - aab.y * m2[j];
-        if (j * kThreads < rec1 && (j + 1) * kThreads > rec0) {  // slot holds receiver cells
-          if (i >= rec0 && i < rec1)                             // gpu.py:180
-            mu = mu + buf_load(rrs, (i - rec0) * 4, it * g.nx * 4);
+        if (j * kAdjThreads < rec1 && (j + 1) * kAdjThreads > rec0) {
+          if (i >= rec0 && i < rec1) mu = mu + buf_load(rrs, (i - rec0) * 4, it * nx * 4);
         }
-        ga[j] = ga[j] + (aab.x * mu) * L;  // gpu.py:181
-        if (j == srci / kThreads) {
-          if (i == srci) sg = sg + (mu * wt) * sdv;  // gpu.py:182
+        ga[j] = ga[j] + (aab.x * mu) * L;
+        if (j == srci / kAdjThreads) {
+          if (i == srci) sg = sg + (mu * wt) * sdv;
         }
-        if (d & kModel) {  // gpu.py:183
-          float &v = il[d & kIdxMask];
+        if (d.model) {
+          float &v = il[d.illum];
           v = v + L * L;
         }
         m2[j] = m1[j];
         m1[j] = mu;
+        asm volatile("" : "+v"(m1[j]), "+v"(m2[j]), "+v"(ga[j]));
       }
-      fence_cells(j);
     }
     __syncthreads();
+    off = t;
+    asm volatile("" : "+v"(off));  // keep per-cell descriptors from being hoisted out of the time loop
 #pragma unroll
     for (int j = 0; j < K; ++j) {
-      const int i = t + j * kThreads;
-      if (i < HW) Q[i] = (desc_of(pk, j) & kInterior) ? div12(ca[j] * m1[j]) : 0.0f;
+      if (t + j * kAdjThreads < HW) {
+        const Desc d = desc(off + j * kAdjThreads);
+        const float q = div12((buf_load(crs, off * 4, j * kAdjThreads * 4) * tab[d.tab].x) * m1[j]);
+        Q[off + j * kAdjThreads] = d.interior ? q : 0.0f;
+      }
     }
     __syncthreads();
   }
 #pragma unroll
   for (int j = 0; j < K; ++j) {
-    const int i = t + j * kThreads;
-    if (i < HW) {
-      m1g[prop * HW + i] = m1[j];
-      m2g[prop * HW + i] = m2[j];
-      gaccg[prop * HW + i] = ga[j];
+    if (t + j * kAdjThreads < HW) {
+      buf_store(m1[j], srs1, t * 4, j * kAdjThreads * 4);
+      buf_store(m2[j], srs2, t * 4, j * kAdjThreads * 4);
+      buf_store(ga[j], srsg, t * 4, j * kAdjThreads * 4);
     }
   }
   if (own_src) src_grad[prop] = sg;
-  for (int k = t; k < nzx; k += kThreads) illumg[prop * nzx + k] = il[k];
+  for (int k = t; k < nzx; k += kAdjThreads) illumg[prop * nzx + k] = il[k];
 }
 
 int device_lds_limit() {
@@ -356,28 +469,41 @@ int device_lds_limit() {
   return limit;
 }
 
-int slots_for(const Geom &g) { return (g.H * g.W + kThreads - 1) / kThreads; }
+int slots_for(const Geom &g, int threads) { return (g.H * g.W + threads - 1) / threads; }
 
-// Shapes compiled with constant dimensions: (slots, H, W).  Everything else
-// runs the runtime-shape instantiation with the smallest slot count >= needed.
+// The OpenFWI geometry (70 x 70 model, pad 40, sources/receivers on row 1)
+// is compiled with its dimensions as constants; everything else runs a
+// runtime-shape instantiation with the smallest slot count >= needed.
+bool is_openfwi(const Geom &g) { return g.nz == 70 && g.nx == 70 && g.pad == 40 && g.row == 1; }
 template <typename Fn>
-int dispatch(const Geom &g, Fn &&fn) {
+int dispatch_fwd(const Geom &g, Fn &&fn) {
   using std::integral_constant;
-  if (g.H == 150 && g.W == 150)
-    return fn(integral_constant<int, 22>(), integral_constant<int, 150>(), integral_constant<int, 150>());
-  const int k = slots_for(g);
-  if (k <= 4) return fn(integral_constant<int, 4>(), integral_constant<int, 0>(), integral_constant<int, 0>());
-  if (k <= 8) return fn(integral_constant<int, 8>(), integral_constant<int, 0>(), integral_constant<int, 0>());
-  if (k <= 16) return fn(integral_constant<int, 16>(), integral_constant<int, 0>(), integral_constant<int, 0>());
-  return fn(integral_constant<int, kMaxSlots>(), integral_constant<int, 0>(), integral_constant<int, 0>());
+  if (is_openfwi(g)) return fn(integral_constant<int, 22>(), std::true_type());
+  // slot counts whose register allocation is spill-free in the time loop
+  const int k = slots_for(g, kFwdThreads);
+  if (k <= 4) return fn(integral_constant<int, 4>(), std::false_type());
+  if (k <= 8) return fn(integral_constant<int, 8>(), std::false_type());
+  return fn(integral_constant<int, kFwdMaxSlots>(), std::false_type());
+}
+
+template <typename Fn>
+int dispatch_adj(const Geom &g, Fn &&fn) {
+  using std::integral_constant;
+  if (is_openfwi(g)) return fn(integral_constant<int, kAdj150>(), std::true_type());
+  const int k = slots_for(g, kAdjThreads);
+  if (k <= 8) return fn(integral_constant<int, 8>(), std::false_type());
+  return fn(integral_constant<int, kAdjMaxSlots>(), std::false_type());
 }
 
 }  // namespace
 
-int64_t resident_max_cells() { return (int64_t)kMaxSlots * kThreads; }
+int64_t resident_max_cells() {
+  const int64_t a = (int64_t)kFwdMaxSlots * kFwdThreads, b = (int64_t)kAdjMaxSlots * kAdjThreads;
+  return a < b ? a : b;
+}
 
 bool resident_fits(const Geom &g) {
-  if (slots_for(g) > kMaxSlots) return false;
+  if (slots_for(g, kFwdThreads) > kFwdMaxSlots || slots_for(g, kAdjThreads) > kAdjMaxSlots) return false;
   if ((g.pad + 1) * (g.pad + 1) > (int)kIdxMask + 1 || g.nz * g.nx > (int)kIdxMask + 1) return false;
   if ((int64_t)g.nt * g.H * g.W > 0x1fffffff) return false;  // history slots addressed with 32-bit byte offsets
   return adj_lds_bytes(g) <= (size_t)device_lds_limit();
@@ -387,12 +513,13 @@ int forward_resident(const Geom &g, int t0, int t1, const float *c2, const float
                      const float *amp, float *p0, float *p1, float *gathers, float *hist, float *ckpt,
                      int ckpt_every, hipStream_t st) {
   const size_t lds = fwd_lds_bytes(g);
-  return dispatch(g, [&](auto kc, auto hc, auto wc) -> int {
-    auto kern = fwd_resident_kernel<decltype(kc)::value, decltype(hc)::value, decltype(wc)::value>;
+  return dispatch_fwd(g, [&](auto kc, auto fixed) -> int {
+    constexpr bool F = decltype(fixed)::value;
+    auto kern = fwd_resident_kernel<decltype(kc)::value, F ? 70 : 0, F ? 70 : 0, F ? 40 : 0, F ? 1 : -1>;
     hipError_t e = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return hip_status(e, "forward_resident attribute");
-    kern<<<dim3((unsigned)((int64_t)g.nb * g.ns)), dim3(kThreads), lds, st>>>(g, t0, t1, c2, A, B, w, amp, p0, p1,
-                                                                              gathers, hist, ckpt, ckpt_every);
+    kern<<<dim3((unsigned)xcd_blocks((int64_t)g.nb * g.ns)), dim3(kFwdThreads), lds, st>>>(g, t0, t1, c2, A, B, w, amp, p0,
+                                                                                 p1, gathers, hist, ckpt, ckpt_every);
     return hip_status(hipGetLastError(), "forward_resident launch");
   });
 }
@@ -401,12 +528,13 @@ int adjoint_resident(const Geom &g, int t0, int t1, const float *c2, const float
                      const float *sd, const float *res, const float *hist, float *m1, float *m2, float *gacc,
                      float *illum, float *src_grad, hipStream_t st) {
   const size_t lds = adj_lds_bytes(g);
-  return dispatch(g, [&](auto kc, auto hc, auto wc) -> int {
-    auto kern = adj_resident_kernel<decltype(kc)::value, decltype(hc)::value, decltype(wc)::value>;
+  return dispatch_adj(g, [&](auto kc, auto fixed) -> int {
+    constexpr bool F = decltype(fixed)::value;
+    auto kern = adj_resident_kernel<decltype(kc)::value, F ? 70 : 0, F ? 70 : 0, F ? 40 : 0, F ? 1 : -1>;
     hipError_t e = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return hip_status(e, "adjoint_resident attribute");
-    kern<<<dim3((unsigned)((int64_t)g.nb * g.ns)), dim3(kThreads), lds, st>>>(g, t0, t1, c2, A, B, w, sd, res, hist,
-                                                                              m1, m2, gacc, illum, src_grad);
+    kern<<<dim3((unsigned)xcd_blocks((int64_t)g.nb * g.ns)), dim3(kAdjThreads), lds, st>>>(g, t0, t1, c2, A, B, w, sd, res,
+                                                                                 hist, m1, m2, gacc, illum, src_grad);
     return hip_status(hipGetLastError(), "adjoint_resident launch");
   });
 }
