@@ -76,6 +76,8 @@ def main():
     ap.add_argument('--seed', type=int, default=0)
     ap.add_argument('--history', default='auto', choices=('auto', 'full', 'checkpoint'))
     ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--cpu-models', type=int, default=8,
+                    help='models in the bounded host-CPU baseline sample (x3 backgrounds x5 shots)')
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) != 1:
         print('bench.py measures one GPU: the path shards by model with no exchange in the time loop; '
@@ -129,7 +131,7 @@ def main():
                      'kernel_ms': avg, 'algorithmic_bytes_per_launch': alg_bytes},
     }
     if not args.no_cpu:
-        rows = 3                               # one model: 3 backgrounds x 5 shots
+        rows = 3 * args.cpu_models             # cpu_models models x 3 backgrounds x 5 shots
         base, cimg, cillum = cpu_baseline(bgs, obs_host, v_mute, rows)
         gimg = img[:rows].cpu().numpy()
         err = float(np.linalg.norm(gimg - cimg) / np.linalg.norm(cimg))
