@@ -320,41 +320,53 @@ class Sim:
             raise ValueError(f'observations must be finite with shape {(nb, g.ns, g.nt, g.nx)}')
         return observed.contiguous()
 
-    def _gradient_device(self, prep, observed=None, obs_repeat=1, v_mute=None, t_mute=0.12, res=None):
-        """Forward, muted residual, exact adjoint and fold on the device.
+    def _forward_tape(self, prep, segment=None, fresh=False):
+        """Forward pass that keeps what the adjoint needs.
 
-        Either ``observed`` (device, (nb/obs_repeat, ns, nt, nx)) or a ready
-        data-space seed ``res`` (nb, ns, nt, nx) is given.  Returns
-        (grad (nb,ns,nz,nx), illum (nb,ns,nz,nx), sim gathers)."""
+        Returns (gathers, tape): tape holds either the whole Laplacian history
+        (one adjoint launch) or checkpoints every ``segment`` steps (the
+        adjoint recomputes each segment, gpu.py:169-176).  ``fresh`` puts the
+        gathers and the tape in new tensors instead of the cached workspaces
+        (for callers that keep them, e.g. autograd)."""
+        g = self.geom
+        nb, c2, amp = prep['nb'], prep['c2'], prep['amp']
+        geom = self._geom(nb)
+        H, W = self.H, self.W
+        seg = self._segment(nb) if segment is None else min(segment, g.nt)
+        new = (lambda name, shape, zero=False: torch.zeros(shape, dtype=self.dtype, device=self.device)) if fresh \
+            else self._scratch
+        p0 = self._scratch('p0', (nb, g.ns, H, W), zero=True)
+        p1 = self._scratch('p1', (nb, g.ns, H, W), zero=True)
+        sim = new('sim', (nb, g.ns, g.nt, g.nx))
+        if seg >= g.nt:
+            hist = new('hist', (nb, g.ns, g.nt, H, W))
+            self._forward(geom, 0, g.nt, c2, amp, p0, p1, gathers=sim, hist=hist)
+            return sim, {'seg': g.nt, 'hist': hist}
+        nck = -(-g.nt // seg)
+        ck = new('ckpt', (nb, g.ns, nck, 2, H, W))
+        self._forward(geom, 0, g.nt, c2, amp, p0, p1, gathers=sim, ckpt=ck, every=seg)
+        return sim, {'seg': seg, 'ckpt': ck}
+
+    def _adjoint_tape(self, prep, tape, res):
+        """Exact adjoint of the taped forward for the data-space seed ``res``
+        (nb, ns, nt, nx), imaging condition and fold (gpu.py:161-188).
+        Returns (grad (nb, ns, nz, nx), illum (nb, ns, nz, nx))."""
         g = self.geom
         nb, c2, amp, sd = prep['nb'], prep['c2'], prep['amp'], prep['sd']
         geom = self._geom(nb)
         H, W = self.H, self.W
-        seg = self._segment(nb)
-        p0 = self._scratch('p0', (nb, g.ns, H, W), zero=True)
-        p1 = self._scratch('p1', (nb, g.ns, H, W), zero=True)
-        sim = self._scratch('sim', (nb, g.ns, g.nt, g.nx))
-        if seg >= g.nt:
-            hist = self._scratch('hist', (nb, g.ns, g.nt, H, W))
-            self._forward(geom, 0, g.nt, c2, amp, p0, p1, gathers=sim, hist=hist)
-            ck = None
-        else:
-            nck = -(-g.nt // seg)
-            ck = self._scratch('ckpt', (nb, g.ns, nck, 2, H, W))
-            self._forward(geom, 0, g.nt, c2, amp, p0, p1, gathers=sim, ckpt=ck, every=seg)
-        if res is None:
-            res = self._scratch('res', (nb, g.ns, g.nt, g.nx))
-            f = _lib.fn('residual', self.dtype)
-            _lib.check(f(geom, _lib.ptr(sim), _lib.ptr(observed), obs_repeat, _lib.ptr(v_mute), g.dt, g.dx, t_mute,
-                         0.03, _lib.ptr(res), self._stream()), 'residual')
         m1 = self._scratch('m1', (nb, g.ns, H, W), zero=True)
         m2 = self._scratch('m2', (nb, g.ns, H, W), zero=True)
         gacc = self._scratch('gacc', (nb, g.ns, H, W), zero=True)
         illum = torch.zeros(nb, g.ns, g.nz, g.nx, dtype=self.dtype, device=self.device)
         sg = torch.zeros(nb, g.ns, dtype=self.dtype, device=self.device)
-        if ck is None:
-            self._adjoint(geom, 0, g.nt, c2, sd, res, hist, m1, m2, gacc, illum, sg)
+        seg = tape['seg']
+        if 'hist' in tape:
+            self._adjoint(geom, 0, g.nt, c2, sd, res, tape['hist'], m1, m2, gacc, illum, sg)
         else:
+            ck = tape['ckpt']
+            p0 = self._scratch('p0', (nb, g.ns, H, W))
+            p1 = self._scratch('p1', (nb, g.ns, H, W))
             hist = self._scratch('hist_seg', (nb, g.ns, seg, H, W))
             for k in reversed(range(ck.shape[2])):
                 t0, t1 = k * seg, min(g.nt, (k + 1) * seg)
@@ -368,6 +380,23 @@ class Sim:
         f = _lib.fn('gradient', self.dtype)
         _lib.check(f(geom, _lib.ptr(gacc), _lib.ptr(prep['v']), _lib.ptr(sg), float(k), _lib.ptr(grad),
                      self._stream()), 'gradient')
+        return grad, illum
+
+    def _gradient_device(self, prep, observed=None, obs_repeat=1, v_mute=None, t_mute=0.12, res=None):
+        """Forward, muted residual, exact adjoint and fold on the device.
+
+        Either ``observed`` (device, (nb/obs_repeat, ns, nt, nx)) or a ready
+        data-space seed ``res`` (nb, ns, nt, nx) is given.  Returns
+        (grad (nb,ns,nz,nx), illum (nb,ns,nz,nx), sim gathers)."""
+        g = self.geom
+        nb = prep['nb']
+        sim, tape = self._forward_tape(prep)
+        if res is None:
+            res = self._scratch('res', (nb, g.ns, g.nt, g.nx))
+            f = _lib.fn('residual', self.dtype)
+            _lib.check(f(self._geom(nb), _lib.ptr(sim), _lib.ptr(observed), obs_repeat, _lib.ptr(v_mute), g.dt,
+                         g.dx, t_mute, 0.03, _lib.ptr(res), self._stream()), 'residual')
+        grad, illum = self._adjoint_tape(prep, tape, res)
         return grad, illum, sim
 
     def _gradient(self, v_bg, obs, v_mute=None, t_mute=0.12):
