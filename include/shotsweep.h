@@ -188,6 +188,40 @@ int ss_lap_adjoint_f64(int64_t n, int H, int W, const double *in, double *out, v
 int ss_fold_edge_padding_f32(int64_t n, int nz, int nx, int pad, const float *g, float *out, void *stream);
 int ss_fold_edge_padding_f64(int64_t n, int nz, int nx, int pad, const double *g, double *out, void *stream);
 
+/*
+ * Learned preconditioner (no reference counterpart: test-time refinement is
+ * "not implemented yet" in the reference, README.md:25).  FP32, NCHW, 3x3
+ * convolutions with zero "same" padding; w is (Cout, Cin, 3, 3), b is (Cout).
+ *   ss_conv3x3_f32        y = conv(x, w) + b, optional ReLU      x (N,Cin,H,W) -> y (N,Cout,H,W)
+ *   ss_conv3x3_dgrad_f32  dx = conv_transpose(dz, w)            dz (N,Cout,H,W) -> dx (N,Cin,H,W)
+ *   ss_conv3x3_wgrad_f32  dw = sum dz x-shifted, db = sum dz   (deterministic reduction order)
+ *   ss_relu_grad_f32      dz = dy * (y > 0)
+ *   ss_mse_f32            grad = (p - t) / n ; partial[k] = block sums of (p - t)^2
+ *   ss_adam_f32           Adam update with bias correction at step >= 1
+ */
+int ss_conv3x3_f32(int N, int Cin, int Cout, int H, int W, const float *x, const float *w, const float *b, float *y,
+                   int relu, void *stream);
+int ss_conv3x3_dgrad_f32(int N, int Cin, int Cout, int H, int W, const float *dz, const float *w, float *dx,
+                         void *stream);
+int ss_conv3x3_wgrad_f32(int N, int Cin, int Cout, int H, int W, const float *x, const float *dz, float *dw,
+                         float *db, void *stream);
+int ss_relu_grad_f32(int64_t n, const float *dy, const float *y, float *dz, void *stream);
+int ss_mse_f32(int64_t n, const float *p, const float *t, float *grad, float *partial, int nblocks, void *stream);
+int ss_adam_f32(int64_t n, float *p, const float *g, float *m, float *v, float lr, float beta1, float beta2,
+                float eps, int step, void *stream);
+/*
+ * Refinement helpers (FP32):
+ *   ss_precond_features_f32  x (B, 2, nz*nx) = (sum_s grad / max|.|, sum_s illum / max|.|), gscale (B) = max|sum_s grad|
+ *                            from grad, illum (B, ns, nz*nx)
+ *   ss_sumsq_diff_f32        out (B) = sum_i (a - c)^2 over each row of n values (data misfit per model, x2)
+ *   ss_velocity_update_f32   out = clamp(v + alpha[b] * d, lo, hi) over B rows of n values
+ */
+int ss_precond_features_f32(int B, int ns, int nzx, const float *grad, const float *illum, float *x, float *gscale,
+                            void *stream);
+int ss_sumsq_diff_f32(int B, int64_t n, const float *a, const float *c, float *out, void *stream);
+int ss_velocity_update_f32(int B, int64_t n, const float *v, const float *d, const float *alpha, float lo, float hi,
+                           float *out, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -53,8 +53,22 @@ def _sig(real):
     }
 
 
+_F = ctypes.c_float
+# FP32-only entry points of the learned preconditioner
+_CNN = {
+    'ss_conv3x3_f32': [_i, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _i, _vp],
+    'ss_conv3x3_dgrad_f32': [_i, _i, _i, _i, _i, _vp, _vp, _vp, _vp],
+    'ss_conv3x3_wgrad_f32': [_i, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp],
+    'ss_relu_grad_f32': [_i64, _vp, _vp, _vp, _vp],
+    'ss_mse_f32': [_i64, _vp, _vp, _vp, _vp, _i, _vp],
+    'ss_adam_f32': [_i64, _vp, _vp, _vp, _vp, _F, _F, _F, _F, _i, _vp],
+    'ss_precond_features_f32': [_i, _i, _i, _vp, _vp, _vp, _vp, _vp],
+    'ss_sumsq_diff_f32': [_i, _i64, _vp, _vp, _vp, _vp],
+    'ss_velocity_update_f32': [_i, _i64, _vp, _vp, _vp, _F, _F, _vp, _vp],
+}
+
 EXPORTS = ['ss_last_error', 'ss_version', 'ss_resident_max_cells'] + [
-    f'ss_{name}_{sfx}' for sfx in ('f32', 'f64') for name in _sig(ctypes.c_float)]
+    f'ss_{name}_{sfx}' for sfx in ('f32', 'f64') for name in _sig(ctypes.c_float)] + list(_CNN)
 
 _LIB = None
 
@@ -77,6 +91,10 @@ def load():
             fn = getattr(lib, f'ss_{name}_{sfx}')
             fn.argtypes = args
             fn.restype = ctypes.c_int
+    for name, args in _CNN.items():
+        fn_ = getattr(lib, name)
+        fn_.argtypes = args
+        fn_.restype = ctypes.c_int
     if lib.ss_version() != ABI_VERSION:
         raise RuntimeError(f'libshotsweep ABI {lib.ss_version()} != expected {ABI_VERSION}; rebuild it')
     _LIB = lib

@@ -353,4 +353,93 @@ int ss_fold_edge_padding_f64(int64_t n, int nz, int nx, int pad, const double *g
   return fold_impl<double>(n, nz, nx, pad, g, out, stream);
 }
 
+static int conv_shape_ok(int N, int Cin, int Cout, int H, int W) {
+  if (N < 1 || Cin < 1 || Cout < 1 || H < 1 || W < 1 || Cin > 32 || Cout > 1024) {
+    set_error("conv3x3: invalid shape (1 <= Cin <= 32)");
+    return 0;
+  }
+  return 1;
+}
+
+int ss_conv3x3_f32(int N, int Cin, int Cout, int H, int W, const float *x, const float *w, const float *b, float *y,
+                   int relu, void *stream) {
+  if (!conv_shape_ok(N, Cin, Cout, H, W)) return SS_EINVAL;
+  if (!x || !w || !y) {
+    set_error("conv3x3: NULL pointer");
+    return SS_EINVAL;
+  }
+  return conv3x3(N, Cin, Cout, H, W, x, w, b, y, relu, 0, (hipStream_t)stream);
+}
+
+int ss_conv3x3_dgrad_f32(int N, int Cin, int Cout, int H, int W, const float *dz, const float *w, float *dx,
+                         void *stream) {
+  if (!conv_shape_ok(N, Cout, Cin, H, W)) return SS_EINVAL;
+  if (!dz || !w || !dx) {
+    set_error("conv3x3_dgrad: NULL pointer");
+    return SS_EINVAL;
+  }
+  return conv3x3(N, Cout, Cin, H, W, dz, w, nullptr, dx, 0, 1, (hipStream_t)stream);
+}
+
+int ss_conv3x3_wgrad_f32(int N, int Cin, int Cout, int H, int W, const float *x, const float *dz, float *dw,
+                         float *db, void *stream) {
+  if (!conv_shape_ok(N, Cin, Cout, H, W)) return SS_EINVAL;
+  if (!x || !dz || !dw) {
+    set_error("conv3x3_wgrad: NULL pointer");
+    return SS_EINVAL;
+  }
+  return conv3x3_wgrad(N, Cin, Cout, H, W, x, dz, dw, db, (hipStream_t)stream);
+}
+
+int ss_relu_grad_f32(int64_t n, const float *dy, const float *y, float *dz, void *stream) {
+  if (n < 0 || (n && (!dy || !y || !dz))) {
+    set_error("relu_grad: invalid arguments");
+    return SS_EINVAL;
+  }
+  return n ? relu_mask(n, dy, y, dz, (hipStream_t)stream) : SS_OK;
+}
+
+int ss_mse_f32(int64_t n, const float *p, const float *t, float *grad, float *partial, int nblocks, void *stream) {
+  if (n < 1 || nblocks < 1 || nblocks > 65536 || !p || !t || !grad || !partial) {
+    set_error("mse: invalid arguments");
+    return SS_EINVAL;
+  }
+  return mse(n, p, t, grad, partial, nblocks, (hipStream_t)stream);
+}
+
+int ss_adam_f32(int64_t n, float *p, const float *g, float *m, float *v, float lr, float beta1, float beta2,
+                float eps, int step, void *stream) {
+  if (n < 1 || step < 1 || !p || !g || !m || !v) {
+    set_error("adam: invalid arguments");
+    return SS_EINVAL;
+  }
+  return adam(n, p, g, m, v, lr, beta1, beta2, eps, step, (hipStream_t)stream);
+}
+
+int ss_precond_features_f32(int B, int ns, int nzx, const float *grad, const float *illum, float *x, float *gscale,
+                            void *stream) {
+  if (B < 1 || ns < 1 || nzx < 1 || !grad || !illum || !x || !gscale) {
+    set_error("precond_features: invalid arguments");
+    return SS_EINVAL;
+  }
+  return precond_features(B, ns, nzx, grad, illum, x, gscale, (hipStream_t)stream);
+}
+
+int ss_sumsq_diff_f32(int B, int64_t n, const float *a, const float *c, float *out, void *stream) {
+  if (B < 1 || n < 1 || !a || !c || !out) {
+    set_error("sumsq_diff: invalid arguments");
+    return SS_EINVAL;
+  }
+  return sumsq_diff(B, n, a, c, out, (hipStream_t)stream);
+}
+
+int ss_velocity_update_f32(int B, int64_t n, const float *v, const float *d, const float *alpha, float lo, float hi,
+                           float *out, void *stream) {
+  if (B < 1 || n < 1 || !v || !d || !alpha || !out || !(lo <= hi)) {
+    set_error("velocity_update: invalid arguments");
+    return SS_EINVAL;
+  }
+  return velocity_update(B, n, v, d, alpha, lo, hi, out, (hipStream_t)stream);
+}
+
 }  // extern "C"

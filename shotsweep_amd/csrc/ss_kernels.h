@@ -33,6 +33,22 @@ int lap_op(int64_t n, int H, int W, const T *in, T *out, int adjoint, hipStream_
 template <typename T>
 int fold_op(int64_t n, int nz, int nx, int pad, const T *g, T *out, hipStream_t st);
 
+// Learned-preconditioner kernels (FP32).
+int conv3x3(int N, int Cin, int Cout, int H, int W, const float *x, const float *w, const float *b, float *y,
+            int relu, int flip, hipStream_t st);
+int conv3x3_wgrad(int N, int Cin, int Cout, int H, int W, const float *x, const float *dz, float *dw, float *db,
+                  hipStream_t st);
+int relu_mask(int64_t n, const float *dy, const float *y, float *dz, hipStream_t st);
+int mse(int64_t n, const float *p, const float *t, float *grad, float *partial, int nblocks, hipStream_t st);
+int adam(int64_t n, float *p, const float *g, float *m, float *v, float lr, float b1, float b2, float eps, int step,
+         hipStream_t st);
+
+int precond_features(int B, int ns, int nzx, const float *grad, const float *illum, float *x, float *gscale,
+                     hipStream_t st);
+int sumsq_diff(int B, int64_t n, const float *a, const float *c, float *out, hipStream_t st);
+int velocity_update(int B, int64_t n, const float *v, const float *d, const float *alpha, float lo, float hi,
+                    float *out, hipStream_t st);
+
 // Resident FP32 kernels (plane held in LDS, one workgroup per propagation).
 int64_t resident_max_cells();
 bool resident_fits(const Geom &g);
