@@ -222,6 +222,61 @@ int ss_sumsq_diff_f32(int B, int64_t n, const float *a, const float *c, float *o
 int ss_velocity_update_f32(int B, int64_t n, const float *v, const float *d, const float *alpha, float lo, float hi,
                            float *out, void *stream);
 
+/*
+ * Mixed-size sweep (FP32).  Up to SS_SWEEP_MAX_GROUPS grid-size groups, each
+ * a batch on its own padded grid with its own step count, shot count and
+ * source columns, advance together: one tiled, temporally blocked launch per
+ * step range covers every active (propagation, tile) item of every group,
+ * and the launch sequence is replayed from a cached hipGraph.  Per cell the
+ * arithmetic is the reference's, as in ss_forward_f32 / ss_adjoint_f32 over
+ * [0, nt) with the whole Laplacian history:
+ *   forward  gpu.py:111-115 and 126-135 (gathers, Laplacian history)
+ *   adjoint  gpu.py:177-184 (imaging sum, illumination, source term)
+ * The reference runs one 70 x 70 geometry per Sim (gpu.py:85, 95, 103, 126);
+ * mixing sizes in one launch has no counterpart there.
+ *
+ * Per group (arrays as in ss_forward_f32 / ss_adjoint_f32, all FP32):
+ *   geom              nb rows x ns shots on (nz+2 pad) x (nx+2 pad), nt steps
+ *   tile_h, tile_w, depth   own-tile shape and steps per launch; 0 = the value
+ *                     ss_sweep_autotune_f32 found for this grid, else a cost model
+ *   c2 (nb,H,W), A, B (H,W), w (nt)
+ *   amp               forward: source amplitude (nb, ns); adjoint: source derivative (nb, ns)
+ *   res               adjoint: muted residual (nb, ns, nt, nx)
+ *   state             scratch, 4 * nb * ns * H * W floats
+ *   gathers           forward output (nb, ns, nt, nx) or NULL
+ *   hist              Laplacian history (nb, ns, nt, H, W): forward writes, adjoint reads
+ *   gacc, illum, src_grad   adjoint outputs (nb,ns,H,W), (nb,ns,nz,nx), (nb,ns), overwritten
+ * Limits: H*W <= 131072, pad <= 44, nt*H*W < 2^29.
+ * use_graph = 1 captures the launch sequence once per layout (group order,
+ * tilings and pointers) and replays it; 0 launches directly.
+ */
+#define SS_SWEEP_MAX_GROUPS 4
+
+typedef struct ss_sweep_group {
+  ss_geom geom;
+  int32_t tile_h, tile_w, depth;
+  const float *c2, *A, *B, *w;
+  const float *amp;
+  const float *res;
+  float *state;
+  float *gathers;
+  float *hist;
+  float *gacc, *illum, *src_grad;
+} ss_sweep_group;
+
+int ss_sweep_forward_f32(int ngroups, const ss_sweep_group *groups, int use_graph, void *stream);
+int ss_sweep_adjoint_f32(int ngroups, const ss_sweep_group *groups, int use_graph, void *stream);
+
+/*
+ * Autotuner: for each group, time up to max_trials candidate tilings (tile
+ * shape and depth ranked by a halo-redundancy model; 0 = 12) as trial
+ * launches of that group alone on its own buffers, write the fastest into
+ * tile_h / tile_w / depth and remember it for the grid size (direction 0 =
+ * forward, 1 = adjoint).  Synchronises the stream; the trial launches leave
+ * the group's outputs and state undefined, so run the sweep afterwards.
+ */
+int ss_sweep_autotune_f32(int ngroups, ss_sweep_group *groups, int direction, int max_trials, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -11,7 +11,7 @@ import torch
 LIB_PATH = Path(__file__).resolve().parent / 'libshotsweep.so'
 MAX_SHOTS = 32
 IMPL_AUTO, IMPL_STEPWISE, IMPL_RESIDENT = 0, 1, 2
-ABI_VERSION = 1
+ABI_VERSION = 2
 
 _vp = ctypes.c_void_p
 _i = ctypes.c_int
@@ -53,6 +53,14 @@ def _sig(real):
     }
 
 
+class SSSweepGroup(ctypes.Structure):
+    """Mirror of ss_sweep_group (one grid-size group of a mixed-size sweep)."""
+    _fields_ = [('geom', SSGeom), ('tile_h', ctypes.c_int32), ('tile_w', ctypes.c_int32), ('depth', ctypes.c_int32),
+                ('c2', _vp), ('A', _vp), ('B', _vp), ('w', _vp), ('amp', _vp), ('res', _vp), ('state', _vp),
+                ('gathers', _vp), ('hist', _vp), ('gacc', _vp), ('illum', _vp), ('src_grad', _vp)]
+
+
+SWEEP_MAX_GROUPS = 4
 _F = ctypes.c_float
 # FP32-only entry points of the learned preconditioner
 _CNN = {
@@ -67,8 +75,14 @@ _CNN = {
     'ss_velocity_update_f32': [_i, _i64, _vp, _vp, _vp, _F, _F, _vp, _vp],
 }
 
+_SWEEP = {
+    'ss_sweep_forward_f32': [_i, ctypes.POINTER(SSSweepGroup), _i, _vp],
+    'ss_sweep_adjoint_f32': [_i, ctypes.POINTER(SSSweepGroup), _i, _vp],
+    'ss_sweep_autotune_f32': [_i, ctypes.POINTER(SSSweepGroup), _i, _i, _vp],
+}
+
 EXPORTS = ['ss_last_error', 'ss_version', 'ss_resident_max_cells'] + [
-    f'ss_{name}_{sfx}' for sfx in ('f32', 'f64') for name in _sig(ctypes.c_float)] + list(_CNN)
+    f'ss_{name}_{sfx}' for sfx in ('f32', 'f64') for name in _sig(ctypes.c_float)] + list(_CNN) + list(_SWEEP)
 
 _LIB = None
 
@@ -91,7 +105,7 @@ def load():
             fn = getattr(lib, f'ss_{name}_{sfx}')
             fn.argtypes = args
             fn.restype = ctypes.c_int
-    for name, args in _CNN.items():
+    for name, args in list(_CNN.items()) + list(_SWEEP.items()):
         fn_ = getattr(lib, name)
         fn_.argtypes = args
         fn_.restype = ctypes.c_int

@@ -1,6 +1,7 @@
 // extern "C" entry points of libshotsweep.so (declared in include/shotsweep.h).
 // Validation happens here; kernels assume a checked geometry.
 #include <cstdio>
+#include <exception>
 #include <string>
 
 #include "ss_kernels.h"
@@ -258,7 +259,7 @@ extern "C" {
 
 const char *ss_last_error(void) { return ss::g_last_error.c_str(); }
 
-int ss_version(void) { return 1; }
+int ss_version(void) { return 2; }
 
 int64_t ss_resident_max_cells(void) { return resident_max_cells(); }
 
@@ -440,6 +441,37 @@ int ss_velocity_update_f32(int B, int64_t n, const float *v, const float *d, con
     return SS_EINVAL;
   }
   return velocity_update(B, n, v, d, alpha, lo, hi, out, (hipStream_t)stream);
+}
+
+int ss_sweep_forward_f32(int ngroups, const ss_sweep_group *groups, int use_graph, void *stream) {
+  try {
+    return sweep_run(ngroups, groups, 0, use_graph, (hipStream_t)stream);
+  } catch (const std::exception &e) {
+    set_error(std::string("sweep forward: ") + e.what());
+    return SS_EHIP;
+  }
+}
+
+int ss_sweep_adjoint_f32(int ngroups, const ss_sweep_group *groups, int use_graph, void *stream) {
+  try {
+    return sweep_run(ngroups, groups, 1, use_graph, (hipStream_t)stream);
+  } catch (const std::exception &e) {
+    set_error(std::string("sweep adjoint: ") + e.what());
+    return SS_EHIP;
+  }
+}
+
+int ss_sweep_autotune_f32(int ngroups, ss_sweep_group *groups, int direction, int max_trials, void *stream) {
+  if (direction != 0 && direction != 1) {
+    set_error("sweep autotune: direction must be 0 (forward) or 1 (adjoint)");
+    return SS_EINVAL;
+  }
+  try {
+    return sweep_autotune(ngroups, groups, direction, max_trials, (hipStream_t)stream);
+  } catch (const std::exception &e) {
+    set_error(std::string("sweep autotune: ") + e.what());
+    return SS_EHIP;
+  }
 }
 
 }  // extern "C"

@@ -1,0 +1,41 @@
+// Device-only helpers shared by the LDS-resident and tiled FP32 kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ss {
+
+constexpr int kXcds = 8;                    // MI355X: 8 XCDs, workgroups dealt round-robin
+constexpr uint32_t kBufWord3 = 0x00020000;  // raw buffer descriptor word 3 for the gfx9 family
+
+// Distance of index i into the absorbing layer of an axis of n cells
+// (gpu.py:86-87: max(pad - i, i - (n - 1 - pad)) clipped at 0).
+__host__ __device__ __forceinline__ int sponge_dist(int i, int n, int pad) {
+  int d = pad - i;
+  int e = i - (n - 1 - pad);
+  d = d > e ? d : e;
+  return d > 0 ? d : 0;
+}
+
+// Buffer resource over nfloats floats at base (range capped at 2^31 - 1 bytes).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const float *base, int64_t nfloats) {
+  const int64_t bytes = nfloats * 4;
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(base), (short)0,
+                                           (int)(bytes > 0x7fffffff ? 0x7fffffff : bytes), (int)kBufWord3);
+}
+
+// Float views of the 32-bit raw buffer intrinsics (which move untyped bits).
+__device__ __forceinline__ void buf_store(float v, __amdgpu_buffer_rsrc_t r, int voff, int soff) {
+  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, voff, soff, 0);
+}
+__device__ __forceinline__ float buf_load(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+
+// floor(n / d) with magic = 0xffffffff / d + 1, exact whenever n * d < 2^32:
+// n * magic / 2^32 exceeds n / d by less than n / 2^32 < 1/d, which is no more
+// than the distance from n / d up to the next integer.
+__device__ __forceinline__ uint32_t magic_div(uint32_t n, uint32_t magic) { return __umulhi(n, magic); }
+
+}  // namespace ss

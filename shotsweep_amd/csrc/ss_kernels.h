@@ -55,6 +55,10 @@ bool resident_fits(const Geom &g);
 int forward_resident(const Geom &g, int t0, int t1, const float *c2, const float *A, const float *B, const float *w,
                      const float *amp, float *p0, float *p1, float *gathers, float *hist, float *ckpt,
                      int ckpt_every, hipStream_t st);
+// Mixed-size tiled sweep (FP32); dir 0 = forward, 1 = adjoint.
+int sweep_run(int ngroups, const ss_sweep_group *groups, int dir, int use_graph, hipStream_t st);
+int sweep_autotune(int ngroups, ss_sweep_group *groups, int dir, int max_trials, hipStream_t st);
+
 int adjoint_resident(const Geom &g, int t0, int t1, const float *c2, const float *A, const float *B, const float *w,
                      const float *sd, const float *res, const float *hist, float *m1, float *m2, float *gacc,
                      float *illum, float *src_grad, hipStream_t st);

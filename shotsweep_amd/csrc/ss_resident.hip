@@ -30,6 +30,7 @@
 // use the same code with runtime dimensions.
 #include <algorithm>
 
+#include "ss_device.h"
 #include "ss_kernels.h"
 
 namespace ss {
@@ -52,7 +53,6 @@ constexpr int kAdjMaxSlots = 48;
 constexpr int kPrefetch = SS_PREFETCH;       // adjoint: Laplacian loads issued this many cells ahead
 // slots of the 150 x 150 plane, rounded up to a multiple of the prefetch distance
 constexpr int kAdj150 = ((150 * 150 + kAdjThreads - 1) / kAdjThreads + kPrefetch - 1) / kPrefetch * kPrefetch;
-constexpr uint32_t kBufWord3 = 0x00020000;  // raw buffer descriptor word 3 for the gfx9 family
 
 // 16-bit cell descriptor
 constexpr uint32_t kIdxMask = 0x1fff;  // sponge table index, or illumination index for model cells
@@ -74,13 +74,6 @@ __device__ __forceinline__ void group_fence(int j, int &off, uint32_t *pk) {
 #pragma unroll
     for (int k = j; k < j + kGroup; k += 2) asm volatile("" : "+v"(pk[k >> 1]));
   }
-}
-
-__device__ __forceinline__ int sponge_dist(int i, int n, int pad) {
-  int d = pad - i;
-  int e = i - (n - 1 - pad);
-  d = d > e ? d : e;
-  return d > 0 ? d : 0;
 }
 
 __device__ __forceinline__ uint32_t cell_desc(int i, int H, int W, int nz, int nx, int pad) {
@@ -128,7 +121,6 @@ __device__ void fill_table(float2 *tab, const float *A, const float *B, int W, i
 // (blockIdx % 8), so XCD x receives blocks x, x+8, ...; mapping those to the
 // contiguous propagation range [x*per, (x+1)*per) keeps the shots of one
 // velocity row (which share their c2 plane) inside one XCD's L2.
-constexpr int kXcds = 8;
 __host__ __device__ __forceinline__ int64_t xcd_blocks(int64_t nprop) {
   return (nprop + kXcds - 1) / kXcds * kXcds;
 }
@@ -162,20 +154,6 @@ __device__ __forceinline__ uint32_t line_desc(int i, int n, int nmodel, int pad)
   if (i >= 2 && i < n - 2) v |= kLineInterior;
   if (i >= pad && i < pad + nmodel) v |= kLineModel | ((uint32_t)(i - pad) << 16);
   return v;
-}
-
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const float *base, int64_t nfloats) {
-  const int64_t bytes = nfloats * 4;
-  return __builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(base), (short)0,
-                                           (int)(bytes > 0x7fffffff ? 0x7fffffff : bytes), (int)kBufWord3);
-}
-
-// Float views of the 32-bit raw buffer intrinsics (which move untyped bits).
-__device__ __forceinline__ void buf_store(float v, __amdgpu_buffer_rsrc_t r, int voff, int soff) {
-  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, voff, soff, 0);
-}
-__device__ __forceinline__ float buf_load(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
-  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
 }
 
 // NZ, NX, PAD > 0 and ROW >= 0 fix the geometry at compile time (0 / -1: runtime).

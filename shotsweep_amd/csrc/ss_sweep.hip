@@ -1,0 +1,742 @@
+// Mixed-size sweep: tiled, temporally blocked FP32 forward and adjoint over
+// several grid-size groups in one launch per step range.
+//
+// A group is a batch of propagations on one padded H x W grid with its own
+// step count, shot count and source columns.  A work item is one tile of one
+// propagation.  One launch advances every active item of every group by its
+// group's temporal depth T: the workgroup loads the item's extended region
+// (the tile plus a halo of 2T cells, clipped to the grid) into LDS and
+// registers and steps it T times.  After step s only cells at least 2s cells
+// inside the region's cut edges are exact; after T steps that is the tile.
+// Halo cells are recomputed redundantly, source and receiver injections
+// included, so every copy of a cell follows the same arithmetic; only the
+// tile's own cells write the Laplacian history, receiver samples, imaging
+// sum and illumination.  The state is double-buffered in HBM between
+// launches (launch k reads buffer k % 2 and writes the other one), and
+// launch 0 starts from zero state and zero accumulators without reading them.
+//
+// Groups with more launches come first in the item list, so the items active
+// in launch k are a prefix of it; groups drop out as their time range ends.
+// The launch sequence is captured once into a hipGraph per batch layout and
+// replayed.  Tile shape and depth per group come from the caller, from the
+// autotuner (trial launches timed with HIP events), or from a cost model.
+//
+// Arithmetic per cell is that of the stepwise kernels (ss_common.h), so the
+// results are bit-identical to them: forward gpu.py:111-115 and 126-135,
+// adjoint gpu.py:177-184.
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "ss_device.h"
+#include "ss_kernels.h"
+
+namespace ss {
+
+namespace {
+
+constexpr int kThreads = 512;
+constexpr int kMaxGroups = SS_SWEEP_MAX_GROUPS;
+constexpr int kFwdMaxK = 32;  // cells per thread: forward keeps p(t-1), c2 and a descriptor
+constexpr int kAdjMaxK = 16;  // adjoint keeps m1, m2, the imaging sum, c2*A and a descriptor
+
+// 32-bit cell descriptor: bits 0-16 offset in the padded plane, bits 17-27
+// sponge table index, then four flags.
+constexpr uint32_t kOffMask = 0x1ffff;
+constexpr int kTabShift = 17;
+constexpr uint32_t kTabMask = 0x7ff;
+constexpr uint32_t kInt = 1u << 28;  // Laplacian interior [2, H-3] x [2, W-3]
+constexpr uint32_t kOwn = 1u << 29;  // inside the item's own tile
+constexpr uint32_t kRow = 1u << 30;  // receiver row, model column (forward: own cells only)
+constexpr uint32_t kSrc = 1u << 31;  // source cell (adjoint: own cell only)
+constexpr int kMaxPlane = 1 << 17;
+constexpr int kMaxTable = 1 << 11;
+
+struct SGroup {
+  const float *c2, *A, *B, *w, *amp, *res;  // amp: forward source amplitude / adjoint source derivative
+  float *st;                                // (2 buffers, 2 planes, nprop, H, W)
+  float *gathers, *hist, *gacc, *illum, *sgrad;
+  int ns, nz, nx, pad, nt, row, H, W;
+  int th, tw, T, ntx, ntiles, nlaunch, item0, nitems;
+  int src[SS_MAX_SHOTS];
+};
+
+struct SArgs {
+  SGroup g[kMaxGroups];
+  int ngroups, k, nactive;
+  int guard, tab_off, il_off;  // LDS layout (floats)
+};
+
+struct ItemView {
+  int prop, b, s;
+  int y0, x0, y1, x1;  // own cells
+  int ey0, ex0, EH, EW;
+};
+
+__device__ __forceinline__ int find_group(const SArgs &a, int item) {
+  int gi = 0;
+#pragma unroll
+  for (int i = 1; i < kMaxGroups; ++i)
+    if (i < a.ngroups && item >= a.g[i].item0) gi = i;
+  return gi;
+}
+
+__device__ __forceinline__ ItemView item_view(const SGroup &G, int item) {
+  ItemView v;
+  const int local = item - G.item0;
+  v.prop = local / G.ntiles;
+  const int tile = local - v.prop * G.ntiles;
+  v.b = v.prop / G.ns;
+  v.s = v.prop - v.b * G.ns;
+  const int ty = tile / G.ntx, tx = tile - ty * G.ntx;
+  v.y0 = ty * G.th;
+  v.x0 = tx * G.tw;
+  v.y1 = min(v.y0 + G.th, G.H);
+  v.x1 = min(v.x0 + G.tw, G.W);
+  v.ey0 = max(0, v.y0 - 2 * G.T);
+  v.ex0 = max(0, v.x0 - 2 * G.T);
+  v.EH = min(G.H, v.y1 + 2 * G.T) - v.ey0;
+  v.EW = min(G.W, v.x1 + 2 * G.T) - v.ex0;
+  return v;
+}
+
+// Workgroup -> item: blocks are dealt round-robin to the 8 XCDs; XCD x takes
+// the contiguous item range [x*per, (x+1)*per), so the tiles of one
+// propagation (which read each other's halos) and the shots of one velocity
+// row (which share c2) stay in one XCD's L2.
+__device__ __forceinline__ int xcd_item() {
+  const int per = (int)gridDim.x / kXcds;
+  return (int)(blockIdx.x % kXcds) * per + (int)(blockIdx.x / kXcds);
+}
+
+__device__ __forceinline__ void sweep_table(float2 *tab, const SGroup &G, int mode) {
+  const int n = (G.pad + 1) * (G.pad + 1);
+  for (int k = threadIdx.x; k < n; k += kThreads) {
+    const int dz = k / (G.pad + 1), dx = k - dz * (G.pad + 1);
+    const int c = (G.pad - dz) * G.W + (G.pad - dx);
+    const float a = G.A[c], b = G.B[c];
+    tab[k] = make_float2(a, mode ? a * b : b);
+  }
+}
+
+__device__ __forceinline__ void zero_guards(float *P, int EHW, int EW) {
+  const int gw = 2 * EW + 2;
+  for (int q = threadIdx.x; q < gw; q += kThreads) {
+    P[-1 - q] = 0.0f;
+    P[EHW + q] = 0.0f;
+  }
+}
+
+template <int K>
+__global__ __launch_bounds__(kThreads) void sweep_fwd_kernel(SArgs a) {
+  extern __shared__ float lds[];
+  const int item = xcd_item();
+  if (item >= a.nactive) return;  // padding workgroup: exits before any barrier
+  const SGroup &G = a.g[find_group(a, item)];
+  const ItemView v = item_view(G, item);
+  const int H = G.H, W = G.W, pad = G.pad, HW = H * W;
+  const int EW = v.EW, EHW = v.EH * v.EW;
+  const int t0 = a.k * G.T, n = min(G.T, G.nt - t0);
+  const int64_t PS = (int64_t)(G.nitems / G.ntiles) * HW;
+  const int ib = a.k & 1;
+  const float *p0i = G.st + 2 * ib * PS + (int64_t)v.prop * HW, *p1i = p0i + PS;
+  float *p0o = G.st + 2 * (ib ^ 1) * PS + (int64_t)v.prop * HW, *p1o = p0o + PS;
+  float *P = lds + a.guard;
+  float2 *tab = reinterpret_cast<float2 *>(lds + a.tab_off);
+  const int t = threadIdx.x;
+  zero_guards(P, EHW, EW);
+  sweep_table(tab, G, 0);
+
+  const uint32_t magic = 0xffffffffu / (uint32_t)EW + 1u;
+  const int r = pad + G.row, sxc = pad + G.src[v.s];
+  const float *c2b = G.c2 + (int64_t)v.b * HW;
+  float pn[K], cc[K];
+  uint32_t dsc[K];
+#pragma unroll
+  for (int j = 0; j < K; ++j) {
+    const int e = t + j * kThreads;
+    pn[j] = 0.0f;
+    cc[j] = 0.0f;
+    dsc[j] = 0;
+    if (e < EHW) {
+      const int q = (int)magic_div((uint32_t)e, magic);
+      const int z = v.ey0 + q, x = v.ex0 + (e - q * EW);
+      const int gi = z * W + x;
+      uint32_t d = (uint32_t)gi | ((uint32_t)(sponge_dist(z, H, pad) * (pad + 1) + sponge_dist(x, W, pad)) << kTabShift);
+      if (z >= 2 && z < H - 2 && x >= 2 && x < W - 2) d |= kInt;
+      const bool own = z >= v.y0 && z < v.y1 && x >= v.x0 && x < v.x1;
+      if (own) d |= kOwn;
+      if (own && z == r && x >= pad && x < pad + G.nx) d |= kRow;
+      if (z == r && x == sxc) d |= kSrc;
+      dsc[j] = d;
+      cc[j] = c2b[gi];
+      if (a.k > 0) {
+        pn[j] = p0i[gi];
+        P[e] = p1i[gi];
+      } else {
+        P[e] = 0.0f;
+      }
+    }
+  }
+  const __amdgpu_buffer_rsrc_t hrs = make_rsrc(G.hist + (int64_t)v.prop * G.nt * HW, (int64_t)G.nt * HW);
+  const __amdgpu_buffer_rsrc_t grs =
+      make_rsrc(G.gathers ? G.gathers + (int64_t)v.prop * G.nt * G.nx : G.hist, (int64_t)G.nt * G.nx);
+  const bool want_gathers = G.gathers != nullptr;
+  const int rec0 = r * W + pad;
+  const float ampv = G.amp[v.prop];
+  __syncthreads();
+
+  for (int st = 0; st < n; ++st) {
+    const int it = t0 + st;
+    const float wsrc = G.w[it] * ampv;
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+      const int e = t + j * kThreads;
+      if (e < EHW) {
+        const uint32_t d = dsc[j];
+        const float c = P[e];
+        const float s = lap_sum(P[e - 2 * EW], P[e - EW], c, P[e + EW], P[e + 2 * EW], P[e - 2], P[e - 1],
+                                P[e + 1], P[e + 2]);
+        const float L = (d & kInt) ? div12(s) : 0.0f;
+        const float2 ab = tab[(d >> kTabShift) & kTabMask];
+        float p2 = leapfrog(c, pn[j], ab.y, cc[j], L, ab.x);
+        if (d & kSrc) p2 = p2 + wsrc;
+        const int gi = (int)(d & kOffMask);
+        if (d & kOwn) buf_store(L, hrs, gi * 4, it * HW * 4);
+        if ((d & kRow) && want_gathers) buf_store(p2, grs, (gi - rec0) * 4, it * G.nx * 4);
+        pn[j] = p2;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+      const int e = t + j * kThreads;
+      if (e < EHW) {
+        const float c = P[e];
+        P[e] = pn[j];
+        pn[j] = c;
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int j = 0; j < K; ++j) {
+    const int e = t + j * kThreads;
+    if (e < EHW && (dsc[j] & kOwn)) {
+      const int gi = (int)(dsc[j] & kOffMask);
+      p0o[gi] = pn[j];
+      p1o[gi] = P[e];
+    }
+  }
+}
+
+template <int K>
+__global__ __launch_bounds__(kThreads) void sweep_adj_kernel(SArgs a) {
+  extern __shared__ float lds[];
+  const int item = xcd_item();
+  if (item >= a.nactive) return;
+  const SGroup &G = a.g[find_group(a, item)];
+  const ItemView v = item_view(G, item);
+  const int H = G.H, W = G.W, pad = G.pad, HW = H * W;
+  const int EW = v.EW, EHW = v.EH * v.EW;
+  const int hi = G.nt - a.k * G.T, n = min(G.T, hi);
+  const int64_t PS = (int64_t)(G.nitems / G.ntiles) * HW;
+  const int ib = a.k & 1;
+  const float *m1i = G.st + 2 * ib * PS + (int64_t)v.prop * HW, *m2i = m1i + PS;
+  float *m1o = G.st + 2 * (ib ^ 1) * PS + (int64_t)v.prop * HW, *m2o = m1o + PS;
+  float *Q = lds + a.guard;
+  float2 *tab = reinterpret_cast<float2 *>(lds + a.tab_off);
+  float *il = lds + a.il_off;
+  const int t = threadIdx.x;
+  zero_guards(Q, EHW, EW);
+  sweep_table(tab, G, 1);
+  __syncthreads();
+
+  const uint32_t magic = 0xffffffffu / (uint32_t)EW + 1u;
+  const int r = pad + G.row, sxc = pad + G.src[v.s];
+  const float *c2b = G.c2 + (int64_t)v.b * HW;
+  float *gac = G.gacc + (int64_t)v.prop * HW;
+  float *ilg = G.illum + (int64_t)v.prop * G.nz * G.nx;
+  float m1[K], m2[K], ga[K], ca[K];
+  uint32_t dsc[K];
+  float sg = 0.0f;
+#pragma unroll
+  for (int j = 0; j < K; ++j) {
+    const int e = t + j * kThreads;
+    m1[j] = m2[j] = ga[j] = ca[j] = 0.0f;
+    dsc[j] = 0;
+    if (e < EHW) {
+      const int q = (int)magic_div((uint32_t)e, magic);
+      const int z = v.ey0 + q, x = v.ex0 + (e - q * EW);
+      const int gi = z * W + x;
+      const uint32_t ti = (uint32_t)(sponge_dist(z, H, pad) * (pad + 1) + sponge_dist(x, W, pad));
+      uint32_t d = (uint32_t)gi | (ti << kTabShift);
+      const bool interior = z >= 2 && z < H - 2 && x >= 2 && x < W - 2;
+      if (interior) d |= kInt;
+      const bool own = z >= v.y0 && z < v.y1 && x >= v.x0 && x < v.x1;
+      const bool model = z >= pad && z < pad + G.nz && x >= pad && x < pad + G.nx;
+      if (own) d |= kOwn;
+      if (z == r && x >= pad && x < pad + G.nx) d |= kRow;
+      if (own && z == r && x == sxc) {
+        d |= kSrc;
+        sg = a.k > 0 ? G.sgrad[v.prop] : 0.0f;
+      }
+      dsc[j] = d;
+      ca[j] = c2b[gi] * tab[ti].x;  // c2 * A (gpu.py:179)
+      float ilv = 0.0f;
+      if (a.k > 0) {
+        m1[j] = m1i[gi];
+        m2[j] = m2i[gi];
+        if (own) ga[j] = gac[gi];
+        if (own && model) ilv = ilg[(z - pad) * G.nx + (x - pad)];
+      }
+      il[e] = ilv;
+      Q[e] = interior ? div12(ca[j] * m1[j]) : 0.0f;
+    }
+  }
+  const __amdgpu_buffer_rsrc_t hrs = make_rsrc(G.hist + (int64_t)v.prop * G.nt * HW, (int64_t)G.nt * HW);
+  const __amdgpu_buffer_rsrc_t rrs = make_rsrc(G.res + (int64_t)v.prop * G.nt * G.nx, (int64_t)G.nt * G.nx);
+  const int rec0 = r * W + pad;
+  const float sdv = G.amp[v.prop];
+  __syncthreads();
+
+  for (int st = 0; st < n; ++st) {
+    const int it = hi - 1 - st;
+    const float wt = G.w[it];
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+      const int e = t + j * kThreads;
+      if (e < EHW) {
+        const uint32_t d = dsc[j];
+        const int gi = (int)(d & kOffMask);
+        const float lt = lapt_sum(Q[e], Q[e + 2 * EW], Q[e + EW], Q[e - EW], Q[e - 2 * EW], Q[e + 2], Q[e + 1],
+                                  Q[e - 1], Q[e - 2]);
+        const float2 aab = tab[(d >> kTabShift) & kTabMask];  // (A, A*B)
+        float mu = (2.0f * aab.x) * m1[j];
+        mu = mu + lt;
+        mu = mu - aab.y * m2[j];
+        if (d & kRow) mu = mu + buf_load(rrs, (gi - rec0) * 4, it * G.nx * 4);
+        if (d & kOwn) {
+          const float L = buf_load(hrs, gi * 4, it * HW * 4);
+          ga[j] = ga[j] + (aab.x * mu) * L;
+          il[e] = il[e] + L * L;
+        }
+        if (d & kSrc) sg = sg + (mu * wt) * sdv;
+        m2[j] = m1[j];
+        m1[j] = mu;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+      const int e = t + j * kThreads;
+      if (e < EHW) Q[e] = (dsc[j] & kInt) ? div12(ca[j] * m1[j]) : 0.0f;
+    }
+    __syncthreads();
+  }
+  const uint32_t magic_w = 0xffffffffu / (uint32_t)W + 1u;
+#pragma unroll
+  for (int j = 0; j < K; ++j) {
+    const int e = t + j * kThreads;
+    const uint32_t d = dsc[j];
+    if (e < EHW && (d & kOwn)) {
+      const int gi = (int)(d & kOffMask);
+      m1o[gi] = m1[j];
+      m2o[gi] = m2[j];
+      gac[gi] = ga[j];
+      const int z = (int)__umulhi((uint32_t)gi, magic_w), x = gi - z * W;
+      if (z >= pad && z < pad + G.nz && x >= pad && x < pad + G.nx) ilg[(z - pad) * G.nx + (x - pad)] = il[e];
+      if (d & kSrc) G.sgrad[v.prop] = sg;
+    }
+  }
+}
+
+// ------------------------------------------------------------------ planning
+
+struct Plan {
+  SArgs args;
+  int dir = 0;  // 0 forward, 1 adjoint
+  int K = 0;
+  size_t lds = 0;
+  int nlaunch = 0;
+  std::vector<int> nactive;
+};
+
+struct TileChoice {
+  int th, tw, T;
+};
+
+int ext_cells(int H, int W, const TileChoice &c) {
+  const int eh = c.th >= H ? H : std::min(H, c.th + 4 * c.T);
+  const int ew = c.tw >= W ? W : std::min(W, c.tw + 4 * c.T);
+  return eh * ew;
+}
+
+// Cells loaded per launch over all tiles, divided by H*W (halo redundancy).
+double redundancy(int H, int W, const TileChoice &c) {
+  int64_t tot = 0;
+  for (int y0 = 0; y0 < H; y0 += c.th) {
+    const int y1 = std::min(H, y0 + c.th);
+    const int eh = std::min(H, y1 + 2 * c.T) - std::max(0, y0 - 2 * c.T);
+    for (int x0 = 0; x0 < W; x0 += c.tw) {
+      const int x1 = std::min(W, x0 + c.tw);
+      tot += (int64_t)eh * (std::min(W, x1 + 2 * c.T) - std::max(0, x0 - 2 * c.T));
+    }
+  }
+  return (double)tot / ((double)H * W);
+}
+
+// Candidate tilings of an H x W grid that fit a workgroup, best modelled first.
+// Model: per step, every loaded cell is updated once, and each launch pays
+// about two extra updates' worth of time for state loads and stores.
+std::vector<TileChoice> candidates(int H, int W, int dir) {
+  const int cap = (dir ? kAdjMaxK : kFwdMaxK) * kThreads;
+  std::vector<std::pair<double, TileChoice>> c;
+  for (int T : {1, 2, 3, 4, 6, 8, 12}) {
+    for (int nty = 1; nty <= 12; ++nty) {
+      const int th = (H + nty - 1) / nty;
+      if (nty > 1 && th < 8) break;
+      for (int ntx = 1; ntx <= 12; ++ntx) {
+        const int tw = (W + ntx - 1) / ntx;
+        if (ntx > 1 && tw < 16) break;
+        const TileChoice tc{th, tw, T};
+        if (ext_cells(H, W, tc) > cap) continue;
+        c.push_back({redundancy(H, W, tc) * (1.0 + 2.0 / T), tc});
+      }
+    }
+  }
+  std::stable_sort(c.begin(), c.end(), [](const auto &x, const auto &y) { return x.first < y.first; });
+  std::vector<TileChoice> out;
+  for (auto &p : c) out.push_back(p.second);
+  return out;
+}
+
+std::mutex g_mutex;
+std::map<std::string, TileChoice> g_tuned;  // "dir:H:W:pad" -> tuned tiling
+
+std::string tune_key(int dir, int H, int W, int pad) {
+  char buf[64];
+  snprintf(buf, sizeof buf, "%d:%d:%d:%d", dir, H, W, pad);
+  return buf;
+}
+
+int fill_group(const ss_sweep_group &in, int dir, SGroup *o, char *err, size_t errn) {
+  Geom g;
+  int rc = check_geom(&in.geom, &g);
+  if (rc != SS_OK) return rc;
+  const int64_t HW = (int64_t)g.H * g.W;
+  if (HW > kMaxPlane || (g.pad + 1) * (g.pad + 1) > kMaxTable || g.nt * HW > (int64_t)0x1fffffff ||
+      (int64_t)g.nt * g.nx > (int64_t)0x1fffffff) {
+    snprintf(err, errn, "sweep: grid %dx%d, pad %d, nt %d exceeds the tiled kernels' limits "
+             "(H*W <= %d, pad <= 44, nt*H*W < 2^29)", g.H, g.W, g.pad, g.nt, kMaxPlane);
+    return SS_EUNSUP;
+  }
+  const void *need[] = {in.c2, in.A, in.B, in.w, in.amp, in.state, in.hist};
+  for (const void *p : need)
+    if (!p) {
+      snprintf(err, errn, "sweep: c2, A, B, w, amp/sd, state and hist are required");
+      return SS_EINVAL;
+    }
+  if (dir && (!in.res || !in.gacc || !in.illum || !in.src_grad)) {
+    snprintf(err, errn, "sweep adjoint: res, gacc, illum and src_grad are required");
+    return SS_EINVAL;
+  }
+  TileChoice tc{in.tile_h, in.tile_w, in.depth};
+  if (tc.th <= 0 || tc.tw <= 0 || tc.T <= 0) {
+    std::lock_guard<std::mutex> lock(g_mutex);
+    auto it = g_tuned.find(tune_key(dir, g.H, g.W, g.pad));
+    if (it != g_tuned.end()) {
+      tc = it->second;
+    } else {
+      const auto c = candidates(g.H, g.W, dir);
+      if (c.empty()) {
+        snprintf(err, errn, "sweep: no tiling of %dx%d fits a workgroup", g.H, g.W);
+        return SS_EUNSUP;
+      }
+      tc = c.front();
+    }
+  }
+  tc.th = std::min(tc.th, g.H);
+  tc.tw = std::min(tc.tw, g.W);
+  const int cap = (dir ? kAdjMaxK : kFwdMaxK) * kThreads;
+  if (ext_cells(g.H, g.W, tc) > cap) {
+    snprintf(err, errn, "sweep: tile %dx%d at depth %d needs %d cells per workgroup (max %d)", tc.th, tc.tw, tc.T,
+             ext_cells(g.H, g.W, tc), cap);
+    return SS_EUNSUP;
+  }
+  const int ntx = (g.W + tc.tw - 1) / tc.tw, nty = (g.H + tc.th - 1) / tc.th;
+  const int64_t nprop = (int64_t)g.nb * g.ns;
+  if (nprop * ntx * nty > (1 << 30)) {
+    snprintf(err, errn, "sweep: too many work items");
+    return SS_EINVAL;
+  }
+  o->c2 = static_cast<const float *>(in.c2);
+  o->A = static_cast<const float *>(in.A);
+  o->B = static_cast<const float *>(in.B);
+  o->w = static_cast<const float *>(in.w);
+  o->amp = static_cast<const float *>(in.amp);
+  o->res = static_cast<const float *>(in.res);
+  o->st = static_cast<float *>(in.state);
+  o->gathers = dir ? nullptr : static_cast<float *>(in.gathers);
+  o->hist = static_cast<float *>(in.hist);
+  o->gacc = static_cast<float *>(in.gacc);
+  o->illum = static_cast<float *>(in.illum);
+  o->sgrad = static_cast<float *>(in.src_grad);
+  o->ns = g.ns;
+  o->nz = g.nz;
+  o->nx = g.nx;
+  o->pad = g.pad;
+  o->nt = g.nt;
+  o->row = g.row;
+  o->H = g.H;
+  o->W = g.W;
+  o->th = tc.th;
+  o->tw = tc.tw;
+  o->T = tc.T;
+  o->ntx = ntx;
+  o->ntiles = ntx * nty;
+  o->nlaunch = (g.nt + tc.T - 1) / tc.T;
+  o->nitems = (int)(nprop * o->ntiles);
+  for (int s = 0; s < SS_MAX_SHOTS; ++s) o->src[s] = g.src[s];
+  return SS_OK;
+}
+
+int make_plan(int ngroups, const ss_sweep_group *groups, int dir, Plan *p) {
+  char err[256];
+  if (ngroups < 1 || ngroups > kMaxGroups || !groups) {
+    snprintf(err, sizeof err, "sweep: need 1..%d groups", kMaxGroups);
+    set_error(err);
+    return SS_EINVAL;
+  }
+  std::vector<SGroup> gs(ngroups);
+  for (int i = 0; i < ngroups; ++i) {
+    err[0] = 0;
+    const int rc = fill_group(groups[i], dir, &gs[i], err, sizeof err);
+    if (rc != SS_OK) {
+      if (err[0]) set_error(std::string(err) + " (group " + std::to_string(i) + ")");
+      return rc;
+    }
+  }
+  std::stable_sort(gs.begin(), gs.end(), [](const SGroup &x, const SGroup &y) { return x.nlaunch > y.nlaunch; });
+  std::memset(&p->args, 0, sizeof p->args);
+  int64_t item0 = 0;
+  int ehw = 0, ew = 0, pad = 0;
+  for (int i = 0; i < ngroups; ++i) {
+    gs[i].item0 = (int)item0;
+    item0 += gs[i].nitems;
+    p->args.g[i] = gs[i];
+    ehw = std::max(ehw, ext_cells(gs[i].H, gs[i].W, {gs[i].th, gs[i].tw, gs[i].T}));
+    ew = std::max(ew, gs[i].tw >= gs[i].W ? gs[i].W : std::min(gs[i].W, gs[i].tw + 4 * gs[i].T));
+    pad = std::max(pad, gs[i].pad);
+  }
+  if (item0 > (1 << 30)) {
+    set_error("sweep: too many work items");
+    return SS_EINVAL;
+  }
+  p->args.ngroups = ngroups;
+  p->dir = dir;
+  p->nlaunch = gs[0].nlaunch;
+  p->nactive.assign(p->nlaunch, 0);
+  for (int k = 0; k < p->nlaunch; ++k)
+    for (int i = 0; i < ngroups; ++i)
+      if (gs[i].nlaunch > k) p->nactive[k] = gs[i].item0 + gs[i].nitems;
+  const int guard = 2 * ew + 2;
+  const int tab_off = (2 * guard + ehw + 1) & ~1;
+  const int il_off = tab_off + 2 * (pad + 1) * (pad + 1);
+  p->args.guard = guard;
+  p->args.tab_off = tab_off;
+  p->args.il_off = il_off;
+  p->lds = (size_t)(il_off + (dir ? ehw : 0)) * sizeof(float);
+  const int kneed = (ehw + kThreads - 1) / kThreads;
+  p->K = kneed <= 8 ? 8 : kneed <= 16 ? 16 : kneed <= 24 ? 24 : 32;
+  return SS_OK;
+}
+
+using KernelFn = void (*)(SArgs);
+
+KernelFn kernel_of(const Plan &p) {
+  if (p.dir == 0) {
+    switch (p.K) {
+      case 8: return sweep_fwd_kernel<8>;
+      case 16: return sweep_fwd_kernel<16>;
+      case 24: return sweep_fwd_kernel<24>;
+      default: return sweep_fwd_kernel<32>;
+    }
+  }
+  return p.K <= 8 ? sweep_adj_kernel<8> : sweep_adj_kernel<16>;
+}
+
+int prepare_kernel(const Plan &p, KernelFn *fn) {
+  *fn = kernel_of(p);
+  int dev = 0, maxlds = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+  if (e != hipSuccess) return hip_status(e, "sweep device query");
+  if ((int64_t)p.lds > maxlds) {
+    set_error("sweep: tiles need more LDS than the device has");
+    return SS_EUNSUP;
+  }
+  return hip_status(hipFuncSetAttribute((const void *)*fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)p.lds),
+                    "sweep attribute");
+}
+
+void launch_range(const Plan &p, KernelFn fn, int k0, int k1, hipStream_t st) {
+  SArgs a = p.args;
+  for (int k = k0; k < k1; ++k) {
+    a.k = k;
+    a.nactive = p.nactive[k];
+    const unsigned grid = (unsigned)((a.nactive + kXcds - 1) / kXcds * kXcds);
+    hipLaunchKernelGGL(fn, dim3(grid), dim3(kThreads), p.lds, st, a);
+  }
+}
+
+// ------------------------------------------------------------------- graphs
+
+struct GraphCache {
+  std::map<std::string, hipGraphExec_t> exec;
+  std::map<int, hipStream_t> capture_stream;
+};
+GraphCache g_graphs;
+
+std::string graph_key(const Plan &p, int dev) {
+  std::string key(reinterpret_cast<const char *>(&p.args), sizeof p.args);
+  key.append(reinterpret_cast<const char *>(&p.dir), sizeof p.dir);
+  key.append(reinterpret_cast<const char *>(&dev), sizeof dev);
+  return key;
+}
+
+int run_plan(const Plan &p, int use_graph, hipStream_t st) {
+  KernelFn fn;
+  int rc = prepare_kernel(p, &fn);
+  if (rc != SS_OK) return rc;
+  if (!use_graph) {
+    launch_range(p, fn, 0, p.nlaunch, st);
+    return hip_status(hipGetLastError(), "sweep launch");
+  }
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_status(e, "sweep device");
+  std::lock_guard<std::mutex> lock(g_mutex);
+  const std::string key = graph_key(p, dev);
+  auto it = g_graphs.exec.find(key);
+  hipGraphExec_t exec = nullptr;
+  if (it != g_graphs.exec.end()) {
+    exec = it->second;
+  } else {
+    hipStream_t cs;
+    auto sit = g_graphs.capture_stream.find(dev);
+    if (sit == g_graphs.capture_stream.end()) {
+      e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking);
+      if (e != hipSuccess) return hip_status(e, "sweep capture stream");
+      g_graphs.capture_stream[dev] = cs;
+    } else {
+      cs = sit->second;
+    }
+    if (g_graphs.exec.size() >= 32) {
+      for (auto &kv : g_graphs.exec) (void)hipGraphExecDestroy(kv.second);
+      g_graphs.exec.clear();
+    }
+    hipGraph_t graph = nullptr;
+    e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return hip_status(e, "sweep begin capture");
+    launch_range(p, fn, 0, p.nlaunch, cs);
+    hipError_t le = hipGetLastError();
+    e = hipStreamEndCapture(cs, &graph);
+    if (le != hipSuccess) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return hip_status(le, "sweep capture launch");
+    }
+    if (e != hipSuccess) return hip_status(e, "sweep end capture");
+    e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) return hip_status(e, "sweep graph instantiate");
+    g_graphs.exec[key] = exec;
+  }
+  return hip_status(hipGraphLaunch(exec, st), "sweep graph launch");
+}
+
+// Time the first (up to) three launches of one group alone under a tiling,
+// after one untimed launch; returns ms per cell-step, negative on error.
+double time_choice(const ss_sweep_group &grp, int dir, const TileChoice &tc, hipStream_t st, hipEvent_t e0,
+                   hipEvent_t e1) {
+  ss_sweep_group g1 = grp;
+  g1.tile_h = tc.th;
+  g1.tile_w = tc.tw;
+  g1.depth = tc.T;
+  Plan p;
+  if (make_plan(1, &g1, dir, &p) != SS_OK) return -1.0;
+  KernelFn fn;
+  if (prepare_kernel(p, &fn) != SS_OK) return -1.0;
+  const SGroup &G = p.args.g[0];
+  const int m = std::min(p.nlaunch, 3);
+  launch_range(p, fn, 0, 1, st);
+  if (hipEventRecord(e0, st) != hipSuccess) return -1.0;
+  launch_range(p, fn, 0, m, st);
+  if (hipEventRecord(e1, st) != hipSuccess) return -1.0;
+  if (hipEventSynchronize(e1) != hipSuccess) return -1.0;
+  float ms = 0.0f;
+  if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess) return -1.0;
+  const double steps = (double)std::min(G.nt, m * G.T);
+  return ms / (steps * (double)G.nitems / G.ntiles * G.H * G.W);
+}
+
+}  // namespace
+
+int sweep_run(int ngroups, const ss_sweep_group *groups, int dir, int use_graph, hipStream_t st) {
+  Plan p;
+  int rc = make_plan(ngroups, groups, dir, &p);
+  if (rc != SS_OK) return rc;
+  return run_plan(p, use_graph, st);
+}
+
+int sweep_autotune(int ngroups, ss_sweep_group *groups, int dir, int max_trials, hipStream_t st) {
+  if (ngroups < 1 || ngroups > kMaxGroups || !groups) {
+    set_error("sweep autotune: need 1..4 groups");
+    return SS_EINVAL;
+  }
+  hipEvent_t e0, e1;
+  hipError_t e = hipEventCreate(&e0);
+  if (e != hipSuccess) return hip_status(e, "autotune event");
+  e = hipEventCreate(&e1);
+  if (e != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return hip_status(e, "autotune event");
+  }
+  int rc = SS_OK;
+  for (int i = 0; i < ngroups && rc == SS_OK; ++i) {
+    Geom g;
+    rc = check_geom(&groups[i].geom, &g);
+    if (rc != SS_OK) break;
+    const auto cand = candidates(g.H, g.W, dir);
+    double best = -1.0;
+    TileChoice pick{0, 0, 0};
+    const int ntry = std::min<int>((int)cand.size(), max_trials > 0 ? max_trials : 12);
+    for (int c = 0; c < ntry; ++c) {
+      const double tpc = time_choice(groups[i], dir, cand[c], st, e0, e1);
+      if (tpc > 0.0 && (best < 0.0 || tpc < best)) {
+        best = tpc;
+        pick = cand[c];
+      }
+    }
+    if (best < 0.0) {
+      set_error("sweep autotune: no candidate tiling ran");
+      rc = SS_EUNSUP;
+      break;
+    }
+    groups[i].tile_h = pick.th;
+    groups[i].tile_w = pick.tw;
+    groups[i].depth = pick.T;
+    std::lock_guard<std::mutex> lock(g_mutex);
+    g_tuned[tune_key(dir, g.H, g.W, g.pad)] = pick;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc == SS_OK) rc = hip_status(hipGetLastError(), "autotune");
+  return rc;
+}
+
+}  // namespace ss

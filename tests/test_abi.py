@@ -52,3 +52,31 @@ def test_geometry_struct_layout():
     assert ctypes.sizeof(_lib.SSGeom) == 4 * (7 + _lib.MAX_SHOTS)
     with pytest.raises(ValueError):
         _lib.make_geom(1, 2, 70, 70, 40, 10, 1, (0,))
+
+
+def test_sweep_group_struct_layout():
+    # ss_geom (39 int32) + 3 int32 tiling fields, then 12 pointers at 8-byte alignment
+    assert ctypes.sizeof(_lib.SSSweepGroup) == 4 * (7 + _lib.MAX_SHOTS) + 12 + 12 * 8
+    assert _lib.SSSweepGroup.c2.offset == 168
+
+
+def test_sweep_validation_without_gpu_work():
+    lib = _lib.load()
+    arr = (_lib.SSSweepGroup * 5)()
+    assert lib.ss_sweep_forward_f32(0, arr, 0, None) == -1
+    assert lib.ss_sweep_forward_f32(5, arr, 0, None) == -1
+    arr[0].geom = _lib.make_geom(2, 3, 20, 30, 8, 50, 1, (0, 15, 29))
+    rc = lib.ss_sweep_forward_f32(1, arr, 0, None)
+    assert rc == -1 and b'required' in lib.ss_last_error()
+    for f in ('c2', 'A', 'B', 'w', 'amp', 'state', 'hist'):
+        setattr(arr[0], f, 64)   # never dereferenced: validation fails first
+    rc = lib.ss_sweep_adjoint_f32(1, arr, 0, None)
+    assert rc == -1 and b'res, gacc' in lib.ss_last_error()
+    arr[0].geom = _lib.make_geom(1, 1, 400, 400, 40, 10, 1, (0,))
+    rc = lib.ss_sweep_forward_f32(1, arr, 0, None)
+    assert rc == -3 and b'limits' in lib.ss_last_error()
+    arr[0].geom = _lib.make_geom(2, 3, 200, 100, 8, 50, 1, (0, 50, 99))
+    arr[0].tile_h, arr[0].tile_w, arr[0].depth = 200, 200, 40
+    rc = lib.ss_sweep_forward_f32(1, arr, 0, None)
+    assert rc == -3 and b'cells per workgroup' in lib.ss_last_error()
+    assert lib.ss_sweep_autotune_f32(1, arr, 2, 0, None) == -1

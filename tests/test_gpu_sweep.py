@@ -1,0 +1,181 @@
+"""Mixed-size tiled sweep (ss_sweep_*_f32) against the stepwise kernels and the oracle.
+
+The tiled kernels recompute halo cells with the same per-cell arithmetic, so
+every output (gathers, Laplacian history, imaging sum, illumination, source
+term, images) must equal the stepwise FP32 path bit for bit, for any tiling:
+tiles that do not divide the grid, depths that do not divide nt, one tile per
+plane and one step per launch.  The stepwise path itself is pinned to the
+reference's golden vectors (test_gpu_parity.py); the images are also checked
+against the oracle at 1e-5 relative L2.
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import fd
+from shotsweep_amd import _lib, gpu, synth
+from shotsweep_amd.constants import Acquisition
+from shotsweep_amd.mixed import ADJOINT, FORWARD, MixedSweep, config_acquisitions
+
+pytestmark = pytest.mark.gpu
+
+DEV = torch.device('cuda')
+PAD = 10
+# long enough (0.23-0.26 s) for the residual to survive the 0.12 s direct-wave mute
+ACQS = [Acquisition(nz=20, nx=30, nt=261, ns=3), Acquisition(nz=26, nx=22, nt=235, ns=2),
+        Acquisition(nz=16, nx=40, nt=250, ns=4, row=3)]
+NB = (2, 3, 1)
+TILINGS = [(8, 12, 1), (13, 17, 3), (60, 60, 2), (11, 64, 4), (30, 9, 5)]
+
+
+def case(name):
+    print(f'\n[case] {name}', flush=True)
+
+
+def rel(a, b):
+    a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+@pytest.fixture(autouse=True)
+def sync_after():
+    yield
+    torch.cuda.synchronize()
+
+
+def models(acq, n, seed):
+    return synth.mixed(n, seed=seed, nz=acq.nz, nx=acq.nx)
+
+
+def stepwise_forward(sim, prep):
+    g = sim.geom
+    nb = prep['nb']
+    p0 = torch.zeros(nb, g.ns, sim.H, sim.W, device=DEV)
+    p1 = torch.zeros_like(p0)
+    out = torch.empty(nb, g.ns, g.nt, g.nx, device=DEV)
+    hist = torch.empty(nb, g.ns, g.nt, sim.H, sim.W, device=DEV)
+    sim._forward(sim._geom(nb), 0, g.nt, prep['c2'], prep['amp'], p0, p1, gathers=out, hist=hist,
+                 impl=_lib.IMPL_STEPWISE)
+    return out, hist
+
+
+def stepwise_adjoint(sim, prep, res, hist):
+    g = sim.geom
+    nb = prep['nb']
+    z = lambda *s: torch.zeros(*s, device=DEV)
+    m1, m2, gacc = z(nb, g.ns, sim.H, sim.W), z(nb, g.ns, sim.H, sim.W), z(nb, g.ns, sim.H, sim.W)
+    illum, sg = z(nb, g.ns, g.nz, g.nx), z(nb, g.ns)
+    sim._adjoint(sim._geom(nb), 0, g.nt, prep['c2'], prep['sd'], res, hist, m1, m2, gacc, illum, sg,
+                 impl=_lib.IMPL_STEPWISE)
+    return gacc, illum, sg
+
+
+def setup_parts(ms, seed=0):
+    parts = []
+    for k, (sim, nb) in enumerate(zip(ms.sims, NB)):
+        prep = sim._prepare(models(sim.geom, nb, seed + k))
+        parts.append((k, prep, ms._buffers(k, nb)))
+    return parts
+
+
+@pytest.mark.parametrize('tiling', TILINGS)
+def test_sweep_forward_and_adjoint_bit_identical_to_stepwise(tiling):
+    case(f'sweep tiling={tiling}')
+    tiles = [{FORWARD: tiling, ADJOINT: tiling} for _ in ACQS]
+    ms = MixedSweep(ACQS, pad=PAD, autotune=False, use_graph=False, tiles=tiles)
+    parts = setup_parts(ms)
+    ms._run(parts, FORWARD)
+    gen = torch.Generator(device=DEV).manual_seed(3)
+    refs = []
+    for k, prep, bufs in parts:
+        sim = ms.sims[k]
+        out, hist = stepwise_forward(sim, prep)
+        assert torch.equal(bufs['sim'], out), f'gathers group {k}'
+        assert torch.equal(bufs['hist'], hist), f'history group {k}'
+        bufs['res'].copy_(torch.randn(bufs['res'].shape, generator=gen, device=DEV))
+        refs.append(stepwise_adjoint(sim, prep, bufs['res'], hist))
+    ms._run(parts, ADJOINT)
+    for (k, prep, bufs), (gacc, illum, sg) in zip(parts, refs):
+        assert torch.equal(bufs['gacc'], gacc), f'imaging sum group {k}'
+        assert torch.equal(bufs['illum'], illum), f'illumination group {k}'
+        assert torch.equal(bufs['sg'], sg), f'source term group {k}'
+
+
+def small_batches(seed=0, rep=2):
+    batches, true = [], []
+    for k, acq in enumerate(ACQS):
+        sim = gpu.Sim(geom=acq, pad=PAD)
+        m = models(acq, NB[k], seed + 10 * k)
+        obs = sim.forward(m)[0].contiguous()
+        bg = np.repeat(synth.smooth(m, 4.0), rep, 0)
+        vm = torch.tensor(np.repeat(m[:, 0].mean(1), rep), dtype=torch.float32, device=DEV)
+        batches.append({'v_bg': bg, 'obs': obs, 'obs_repeat': rep, 'v_mute': vm})
+        true.append(m)
+    return batches, true
+
+
+def test_mixed_images_equal_single_size_sims_and_oracle():
+    case('mixed images vs Sim and oracle')
+    batches, _ = small_batches()
+    ms = MixedSweep(ACQS, pad=PAD)
+    out = ms.images_device(batches)
+    for k, (acq, b) in enumerate(zip(ACQS, batches)):
+        sim = gpu.Sim(geom=acq, pad=PAD, impl=_lib.IMPL_STEPWISE)
+        prep = sim._prepare(b['v_bg'])
+        img, illum = sim.images_device(prep, b['obs'], b['obs_repeat'], b['v_mute'])
+        np.testing.assert_array_equal(out[k][0].cpu().numpy(), img.cpu().numpy())
+        np.testing.assert_array_equal(out[k][1].cpu().numpy(), illum.cpu().numpy())
+        osim = fd.OracleSim(pad=PAD, geom=fd.Geometry(nz=acq.nz, nx=acq.nx, nt=acq.nt, ns=acq.ns, row=acq.row))
+        obs_rep = np.repeat(b['obs'].cpu().numpy(), b['obs_repeat'], 0)
+        oimg, oill = osim.images(b['v_bg'], obs_rep, v_mute=b['v_mute'].cpu().numpy())
+        assert np.abs(oimg).max() > 0
+        assert rel(out[k][0].cpu().numpy(), oimg) < 1e-5
+        np.testing.assert_array_equal(out[k][1].cpu().numpy(), oill)
+    for k in range(len(ACQS)):
+        th, tw, d = ms.tiles[k][FORWARD]
+        assert th >= 1 and tw >= 1 and d >= 1
+
+
+def test_graph_replays_and_chunks_match_direct_launches():
+    case('graph replay / chunking')
+    batches, _ = small_batches(seed=5)
+    ref = MixedSweep(ACQS, pad=PAD, autotune=False, use_graph=False).images_device(batches)
+    ms = MixedSweep(ACQS, pad=PAD, autotune=False, use_graph=True)
+    for rep in range(3):
+        out = ms.images_device(batches)
+        x = torch.randn(1000, device=DEV)
+        _ = float((x * x).sum())       # eager work between replays
+        torch.cuda.synchronize()
+        for k in range(len(ACQS)):
+            assert torch.equal(out[k][0], ref[k][0]), f'replay {rep} group {k}'
+            assert torch.equal(out[k][1], ref[k][1])
+    small = MixedSweep(ACQS, pad=PAD, autotune=False, max_history_bytes=ms.history_bytes([4, 6, 2]) // 3)
+    n, _ = small._chunks([4, 6, 2], [2, 2, 2])
+    assert n >= 2
+    out = small.images_device(batches)
+    for k in range(len(ACQS)):
+        assert torch.equal(out[k][0], ref[k][0])
+        assert torch.equal(out[k][1], ref[k][1])
+
+
+def test_uneven_workload_sizes_bit_identical_to_stepwise():
+    """One model per grid size of the uneven-batch workload (70x70/100x140/200x200,
+    nt 1000/1500/2000, 5/7/10 shots, pad 40) under the autotuned tilings."""
+    case('uneven workload sizes vs stepwise')
+    acqs = config_acquisitions()
+    batches = []
+    for k, acq in enumerate(acqs):
+        sim = gpu.Sim(geom=acq)
+        m = models(acq, 1, 40 + k)
+        obs = sim.forward(m)[0].contiguous()
+        vm = torch.tensor(m[:, 0].mean(1), dtype=torch.float32, device=DEV)
+        batches.append({'v_bg': synth.smooth(m, 8.0), 'obs': obs, 'obs_repeat': 1, 'v_mute': vm})
+    ms = MixedSweep(acqs)
+    out = ms.images_device(batches)
+    for k, (acq, b) in enumerate(zip(acqs, batches)):
+        sim = gpu.Sim(geom=acq, impl=_lib.IMPL_STEPWISE)
+        prep = sim._prepare(b['v_bg'])
+        img, illum = sim.images_device(prep, b['obs'], 1, b['v_mute'])
+        print(f'  group {k}: tiles {ms.tiles[k]}', flush=True)
+        assert torch.equal(out[k][0], img), f'images group {k}'
+        assert torch.equal(out[k][1], illum), f'illumination group {k}'
