@@ -1,6 +1,7 @@
 """Benchmark: batched forward + exact-adjoint RTM images on one MI355X.
 
     python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--models 64] [--seed 0] [--no-cpu]
+    python bench.py --workload mixed [--models-per-size 32]    (uneven batch, see run_mixed)
 
 Workload (one "step"): 64 seeded 70 x 70 models of mixed structure (curved
 layers, a planar fault, smoothed texture) x 3 backgrounds (surface velocity +
@@ -67,6 +68,108 @@ def cpu_baseline(bgs, obs_host, v_mute, n_rows):
                       f'eager PyTorch on {cores} threads, one run of {dt:.1f} s'}, img, illum
 
 
+def mixed_workload(n_per_size, seed, acqs):
+    """Per grid size: models, observed gathers (device), backgrounds and mute velocities.
+
+    Surface velocity = mean of the model's top row (the first-break picker is
+    specific to the 70-receiver acquisition); trend = depth_trend of the size's
+    models; 3 backgrounds per model as in the batched workload."""
+    from shotsweep_amd import background as BG, gpu, synth
+    out = []
+    for k, acq in enumerate(acqs):
+        sim = gpu.Sim(geom=acq)
+        models = synth.mixed(n_per_size, seed=seed + 1000 * k, nz=acq.nz, nx=acq.nx)
+        obs = torch.cat([sim.forward(models[i:i + 8])[0] for i in range(0, n_per_size, 8)]).contiguous()
+        sim.release()
+        trend = BG.depth_trend(models)
+        v_s = models[:, 0].mean(1)
+        bgs = np.concatenate([BG.backgrounds(v, trend, nx=acq.nx) for v in v_s])
+        vm = np.repeat(v_s, len(BG.RHOS))
+        out.append({'v_bg': bgs, 'obs': obs, 'obs_repeat': len(BG.RHOS),
+                    'v_mute': torch.tensor(vm, dtype=torch.float32, device='cuda')})
+    return out
+
+
+def mixed_cpu_baseline(acqs, batches, shot_images):
+    """Oracle on one background row per grid size, extrapolated to the whole workload."""
+    from oracle import fd
+    cores = len(os.sched_getaffinity(0))
+    torch.set_num_threads(cores)
+    total, parts, imgs = 0.0, [], []
+    for acq, b in zip(acqs, batches):
+        osim = fd.OracleSim(geom=fd.Geometry(nz=acq.nz, nx=acq.nx, nt=acq.nt, ns=acq.ns, row=acq.row))
+        obs = b['obs'][:1].cpu().numpy()
+        t0 = time.perf_counter()
+        img, illum = osim.images(b['v_bg'][:1], obs, v_mute=b['v_mute'][:1].cpu().numpy())
+        dt = time.perf_counter() - t0
+        total += dt * len(b['v_bg'])
+        parts.append(f'{acq.nz}x{acq.nx}: {dt:.1f} s')
+        imgs.append((img, illum))
+    return {'value': shot_images / total, 'unit': 'shot-images/s', 'cores': cores, 'kind': 'port',
+            'sample': 'one background row per grid size (oracle FP32 eager PyTorch on '
+                      f'{cores} threads; {", ".join(parts)}), extrapolated linearly to every row'}, imgs
+
+
+def run_mixed(args):
+    """Uneven batch: 32 models each of 70x70 (nt 1000, 5 shots), 100x140 (nt 1500,
+    7 shots) and 200x200 (nt 2000, 10 shots) x 3 backgrounds in one mixed sweep."""
+    from shotsweep_amd.mixed import MixedSweep, config_acquisitions
+    acqs = config_acquisitions()
+    batches = mixed_workload(args.models_per_size, args.seed, acqs)
+    ms = MixedSweep(acqs)
+    shot_images = sum(len(b['v_bg']) * a.ns for a, b in zip(acqs, batches))
+    rows = [len(b['v_bg']) for b in batches]
+    nchunks, _ = ms._chunks(rows, [b['obs_repeat'] for b in batches])
+    print(f'[bench] mixed: rows {rows}, {shot_images} shot-images, {nchunks} chunks', file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    for _ in range(args.warmup):
+        ms.images_device(batches)
+    torch.cuda.synchronize()
+    print(f'[bench] warmup (with autotuning) {time.perf_counter() - t0:.1f} s, tiles {ms.tiles}',
+          file=sys.stderr, flush=True)
+    ms.events = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        out = ms.images_device(batches)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    events, ms.events = ms.events, None
+    ms_per_step = wall / args.steps * 1e3
+    value = shot_images / (wall / args.steps)
+    per = {}
+    for name, a, b in events:
+        per[name] = per.get(name, 0.0) + a.elapsed_time(b) / args.steps
+    dominant = max(per, key=per.get)
+    cell_updates = sum(len(b['v_bg']) * a.ns * a.nt * (a.nz + 80) * (a.nx + 80) for a, b in zip(acqs, batches))
+    alg_bytes = BYTES_PER_CELL_UPDATE * cell_updates
+    achieved = alg_bytes / (per[dominant] * 1e-3) / 1e9
+    print(f'[bench] {ms_per_step:.1f} ms/step, sweep ms per step {per}', file=sys.stderr, flush=True)
+    result = {
+        'metric': METRIC, 'value': value, 'unit': 'shot-images/s', 'n_gpus': 1, 'steps': args.steps,
+        'warmup': args.warmup, 'ms_per_step': ms_per_step, 'higher_is_better': True, 'scaling': 'weak',
+        'vs_baseline': None, 'dtype': 'f32', 'data': 'synthetic (seeded mixed-structure models at each size)',
+        'config': {'workload': 'uneven batched RTM: 32 models each of 70x70 (nt 1000, 5 shots), 100x140 '
+                               '(nt 1500, 7 shots), 200x200 (nt 2000, 10 shots) x 3 backgrounds, pad 40, '
+                               'dx 10 m, 15 Hz Ricker', 'models_per_size': args.models_per_size,
+                   'shot_images': shot_images, 'seed': args.seed, 'chunks': nchunks,
+                   'tiles': {f'{a.nz}x{a.nx}': {('forward', 'adjoint')[d]: t for d, t in ms.tiles[k].items()}
+                             for k, a in enumerate(acqs)}},
+        'roofline': {'bound': 'hbm', 'kernel': dominant, 'achieved': achieved, 'peak': PEAK_HBM_GBS, 'unit': 'GB/s',
+                     'frac': achieved / PEAK_HBM_GBS, 'traffic': None, 'sweep_ms_per_step': per,
+                     'algorithmic_bytes_per_direction': alg_bytes},
+    }
+    if not args.no_cpu:
+        base, imgs = mixed_cpu_baseline(acqs, batches, shot_images)
+        result['cpu_baseline'] = base
+        result['speedup_vs_cpu'] = value / base['value']
+        errs = []
+        for (cimg, cill), (gimg, gill) in zip(imgs, out):
+            g = gimg[:1].cpu().numpy()
+            errs.append(float(np.linalg.norm(g - cimg) / np.linalg.norm(cimg)))
+        result['image_rel_l2_vs_cpu_oracle'] = max(errs)
+    print(json.dumps(result), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -78,11 +181,16 @@ def main():
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--cpu-models', type=int, default=8,
                     help='models in the bounded host-CPU baseline sample (x3 backgrounds x5 shots)')
+    ap.add_argument('--workload', default='batched', choices=('batched', 'mixed'),
+                    help='batched: 64 models of 70x70 (the headline metric); mixed: the uneven batch')
+    ap.add_argument('--models-per-size', type=int, default=32)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) != 1:
         print('bench.py measures one GPU: the path shards by model with no exchange in the time loop; '
               'multi-GPU runs are not part of this build', file=sys.stderr)
         sys.exit(2)
+    if args.workload == 'mixed':
+        return run_mixed(args)
     from shotsweep_amd import gpu
 
     dev = torch.device('cuda')

@@ -39,22 +39,27 @@ namespace ss {
 
 namespace {
 
-constexpr int kThreads = 512;
+constexpr int kThreads = 1024;
 constexpr int kMaxGroups = SS_SWEEP_MAX_GROUPS;
-constexpr int kFwdMaxK = 32;  // cells per thread: forward keeps p(t-1), c2 and a descriptor
-constexpr int kAdjMaxK = 16;  // adjoint keeps m1, m2, the imaging sum, c2*A and a descriptor
+constexpr int kFwdMaxK = 16;  // cells per thread: forward keeps p(t-1), c2 and a descriptor
+constexpr int kAdjMaxK = 12;  // adjoint keeps m1, m2, the imaging sum, c2*A and a descriptor
+constexpr int kMaxLdsFloats = 160 * 1024 / 4;
 
 // 32-bit cell descriptor: bits 0-16 offset in the padded plane, bits 17-27
-// sponge table index, then four flags.
+// sponge table index, then flags.
 constexpr uint32_t kOffMask = 0x1ffff;
 constexpr int kTabShift = 17;
 constexpr uint32_t kTabMask = 0x7ff;
 constexpr uint32_t kInt = 1u << 28;  // Laplacian interior [2, H-3] x [2, W-3]
 constexpr uint32_t kOwn = 1u << 29;  // inside the item's own tile
-constexpr uint32_t kRow = 1u << 30;  // receiver row, model column (forward: own cells only)
-constexpr uint32_t kSrc = 1u << 31;  // source cell (adjoint: own cell only)
+constexpr uint32_t kRow = 1u << 30;  // adjoint: receiver row, model column (every copy)
+constexpr uint32_t kSrc = 1u << 31;  // source cell (forward: every copy; adjoint: own cell)
 constexpr int kMaxPlane = 1 << 17;
 constexpr int kMaxTable = 1 << 11;
+// A raw buffer access whose offset is at least the resource's size is
+// discarded (stores) or reads zero (loads).  Every resource here covers fewer
+// than 2^31 bytes, so this offset masks an access without a branch.
+constexpr int kNoAccess = (int)0x80000000u;
 
 struct SGroup {
   const float *c2, *A, *B, *w, *amp, *res;  // amp: forward source amplitude / adjoint source derivative
@@ -65,10 +70,11 @@ struct SGroup {
   int src[SS_MAX_SHOTS];
 };
 
+// LDS layout (floats): [guard | plane 0 | guard | plane 1 | guard | sponge table | illumination]
 struct SArgs {
   SGroup g[kMaxGroups];
   int ngroups, k, nactive;
-  int guard, tab_off, il_off;  // LDS layout (floats)
+  int plane0, plane1, tab_off, il_off;
 };
 
 struct ItemView {
@@ -123,14 +129,21 @@ __device__ __forceinline__ void sweep_table(float2 *tab, const SGroup &G, int mo
   }
 }
 
-__device__ __forceinline__ void zero_guards(float *P, int EHW, int EW) {
+// Zero the stencil overhang (2 rows + 2 cells) before and after both planes.
+__device__ __forceinline__ void zero_guards(float *P0, float *P1, int EHW, int EW) {
   const int gw = 2 * EW + 2;
   for (int q = threadIdx.x; q < gw; q += kThreads) {
-    P[-1 - q] = 0.0f;
-    P[EHW + q] = 0.0f;
+    P0[-1 - q] = 0.0f;
+    P0[EHW + q] = 0.0f;
+    P1[-1 - q] = 0.0f;
+    P1[EHW + q] = 0.0f;
   }
 }
 
+// Forward: plane Pc holds p(t); each thread keeps p(t-1), c2 and descriptors
+// of its cells in registers and writes p(t+1) to the other plane, so one
+// barrier per step suffices.  Receiver samples of the own tile are copied
+// from LDS by the first threads at the start of the next step.
 template <int K>
 __global__ __launch_bounds__(kThreads) void sweep_fwd_kernel(SArgs a) {
   extern __shared__ float lds[];
@@ -145,10 +158,10 @@ __global__ __launch_bounds__(kThreads) void sweep_fwd_kernel(SArgs a) {
   const int ib = a.k & 1;
   const float *p0i = G.st + 2 * ib * PS + (int64_t)v.prop * HW, *p1i = p0i + PS;
   float *p0o = G.st + 2 * (ib ^ 1) * PS + (int64_t)v.prop * HW, *p1o = p0o + PS;
-  float *P = lds + a.guard;
+  float *Pc = lds + a.plane0, *Pn = lds + a.plane1;
   float2 *tab = reinterpret_cast<float2 *>(lds + a.tab_off);
   const int t = threadIdx.x;
-  zero_guards(P, EHW, EW);
+  zero_guards(Pc, Pn, EHW, EW);
   sweep_table(tab, G, 0);
 
   const uint32_t magic = 0xffffffffu / (uint32_t)EW + 1u;
@@ -168,72 +181,69 @@ __global__ __launch_bounds__(kThreads) void sweep_fwd_kernel(SArgs a) {
       const int gi = z * W + x;
       uint32_t d = (uint32_t)gi | ((uint32_t)(sponge_dist(z, H, pad) * (pad + 1) + sponge_dist(x, W, pad)) << kTabShift);
       if (z >= 2 && z < H - 2 && x >= 2 && x < W - 2) d |= kInt;
-      const bool own = z >= v.y0 && z < v.y1 && x >= v.x0 && x < v.x1;
-      if (own) d |= kOwn;
-      if (own && z == r && x >= pad && x < pad + G.nx) d |= kRow;
+      if (z >= v.y0 && z < v.y1 && x >= v.x0 && x < v.x1) d |= kOwn;
       if (z == r && x == sxc) d |= kSrc;
       dsc[j] = d;
       cc[j] = c2b[gi];
       if (a.k > 0) {
         pn[j] = p0i[gi];
-        P[e] = p1i[gi];
+        Pc[e] = p1i[gi];
       } else {
-        P[e] = 0.0f;
+        Pc[e] = 0.0f;
       }
     }
   }
   const __amdgpu_buffer_rsrc_t hrs = make_rsrc(G.hist + (int64_t)v.prop * G.nt * HW, (int64_t)G.nt * HW);
-  const __amdgpu_buffer_rsrc_t grs =
-      make_rsrc(G.gathers ? G.gathers + (int64_t)v.prop * G.nt * G.nx : G.hist, (int64_t)G.nt * G.nx);
-  const bool want_gathers = G.gathers != nullptr;
-  const int rec0 = r * W + pad;
+  // own receivers: row r, model columns [xa, xb) of this tile
+  const int xa = max(v.x0, pad), xb = min(v.x1, pad + G.nx);
+  const bool rec = G.gathers != nullptr && r >= v.y0 && r < v.y1 && t < xb - xa;
+  const int rec_e = (r - v.ey0) * EW + (xa + t - v.ex0);
+  float *grec = G.gathers ? G.gathers + (int64_t)v.prop * G.nt * G.nx + (xa - pad) + t : nullptr;
   const float ampv = G.amp[v.prop];
   __syncthreads();
 
   for (int st = 0; st < n; ++st) {
     const int it = t0 + st;
+    if (rec && st > 0) grec[(int64_t)(it - 1) * G.nx] = Pc[rec_e];  // p(it) is the sample of step it-1
     const float wsrc = G.w[it] * ampv;
 #pragma unroll
     for (int j = 0; j < K; ++j) {
       const int e = t + j * kThreads;
       if (e < EHW) {
         const uint32_t d = dsc[j];
-        const float c = P[e];
-        const float s = lap_sum(P[e - 2 * EW], P[e - EW], c, P[e + EW], P[e + 2 * EW], P[e - 2], P[e - 1],
-                                P[e + 1], P[e + 2]);
+        const float c = Pc[e];
+        const float s = lap_sum(Pc[e - 2 * EW], Pc[e - EW], c, Pc[e + EW], Pc[e + 2 * EW], Pc[e - 2], Pc[e - 1],
+                                Pc[e + 1], Pc[e + 2]);
         const float L = (d & kInt) ? div12(s) : 0.0f;
         const float2 ab = tab[(d >> kTabShift) & kTabMask];
         float p2 = leapfrog(c, pn[j], ab.y, cc[j], L, ab.x);
-        if (d & kSrc) p2 = p2 + wsrc;
-        const int gi = (int)(d & kOffMask);
-        if (d & kOwn) buf_store(L, hrs, gi * 4, it * HW * 4);
-        if ((d & kRow) && want_gathers) buf_store(p2, grs, (gi - rec0) * 4, it * G.nx * 4);
-        pn[j] = p2;
-      }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int j = 0; j < K; ++j) {
-      const int e = t + j * kThreads;
-      if (e < EHW) {
-        const float c = P[e];
-        P[e] = pn[j];
+        const float ps = p2 + wsrc;
+        p2 = (d & kSrc) ? ps : p2;
+        buf_store(L, hrs, (d & kOwn) ? (int)(d & kOffMask) * 4 : kNoAccess, it * HW * 4);
+        Pn[e] = p2;
         pn[j] = c;
       }
     }
     __syncthreads();
+    float *tmp = Pc;
+    Pc = Pn;
+    Pn = tmp;
   }
+  if (rec) grec[(int64_t)(t0 + n - 1) * G.nx] = Pc[rec_e];
 #pragma unroll
   for (int j = 0; j < K; ++j) {
     const int e = t + j * kThreads;
     if (e < EHW && (dsc[j] & kOwn)) {
       const int gi = (int)(dsc[j] & kOffMask);
       p0o[gi] = pn[j];
-      p1o[gi] = P[e];
+      p1o[gi] = Pc[e];
     }
   }
 }
 
+// Adjoint: plane Qc holds q = (c2 A m1)/12 on the interior (zero on the
+// two-cell border); each thread keeps m1, m2, c2*A, the imaging sum and
+// descriptors in registers and writes the next q to the other plane.
 template <int K>
 __global__ __launch_bounds__(kThreads) void sweep_adj_kernel(SArgs a) {
   extern __shared__ float lds[];
@@ -248,11 +258,11 @@ __global__ __launch_bounds__(kThreads) void sweep_adj_kernel(SArgs a) {
   const int ib = a.k & 1;
   const float *m1i = G.st + 2 * ib * PS + (int64_t)v.prop * HW, *m2i = m1i + PS;
   float *m1o = G.st + 2 * (ib ^ 1) * PS + (int64_t)v.prop * HW, *m2o = m1o + PS;
-  float *Q = lds + a.guard;
+  float *Qc = lds + a.plane0, *Qn = lds + a.plane1;
   float2 *tab = reinterpret_cast<float2 *>(lds + a.tab_off);
   float *il = lds + a.il_off;
   const int t = threadIdx.x;
-  zero_guards(Q, EHW, EW);
+  zero_guards(Qc, Qn, EHW, EW);
   sweep_table(tab, G, 1);
   __syncthreads();
 
@@ -295,7 +305,7 @@ __global__ __launch_bounds__(kThreads) void sweep_adj_kernel(SArgs a) {
         if (own && model) ilv = ilg[(z - pad) * G.nx + (x - pad)];
       }
       il[e] = ilv;
-      Q[e] = interior ? div12(ca[j] * m1[j]) : 0.0f;
+      Qc[e] = interior ? div12(ca[j] * m1[j]) : 0.0f;
     }
   }
   const __amdgpu_buffer_rsrc_t hrs = make_rsrc(G.hist + (int64_t)v.prop * G.nt * HW, (int64_t)G.nt * HW);
@@ -313,30 +323,27 @@ __global__ __launch_bounds__(kThreads) void sweep_adj_kernel(SArgs a) {
       if (e < EHW) {
         const uint32_t d = dsc[j];
         const int gi = (int)(d & kOffMask);
-        const float lt = lapt_sum(Q[e], Q[e + 2 * EW], Q[e + EW], Q[e - EW], Q[e - 2 * EW], Q[e + 2], Q[e + 1],
-                                  Q[e - 1], Q[e - 2]);
+        const float L = buf_load(hrs, (d & kOwn) ? gi * 4 : kNoAccess, it * HW * 4);  // 0 off the own tile
+        const float lt = lapt_sum(Qc[e], Qc[e + 2 * EW], Qc[e + EW], Qc[e - EW], Qc[e - 2 * EW], Qc[e + 2],
+                                  Qc[e + 1], Qc[e - 1], Qc[e - 2]);
         const float2 aab = tab[(d >> kTabShift) & kTabMask];  // (A, A*B)
         float mu = (2.0f * aab.x) * m1[j];
         mu = mu + lt;
         mu = mu - aab.y * m2[j];
         if (d & kRow) mu = mu + buf_load(rrs, (gi - rec0) * 4, it * G.nx * 4);
-        if (d & kOwn) {
-          const float L = buf_load(hrs, gi * 4, it * HW * 4);
-          ga[j] = ga[j] + (aab.x * mu) * L;
-          il[e] = il[e] + L * L;
-        }
+        const float gn = ga[j] + (aab.x * mu) * L;
+        ga[j] = (d & kOwn) ? gn : ga[j];
+        il[e] = il[e] + L * L;  // unchanged off the own tile (L = 0)
         if (d & kSrc) sg = sg + (mu * wt) * sdv;
         m2[j] = m1[j];
         m1[j] = mu;
+        Qn[e] = (d & kInt) ? div12(ca[j] * mu) : 0.0f;
       }
     }
     __syncthreads();
-#pragma unroll
-    for (int j = 0; j < K; ++j) {
-      const int e = t + j * kThreads;
-      if (e < EHW) Q[e] = (dsc[j] & kInt) ? div12(ca[j] * m1[j]) : 0.0f;
-    }
-    __syncthreads();
+    float *tmp = Qc;
+    Qc = Qn;
+    Qn = tmp;
   }
   const uint32_t magic_w = 0xffffffffu / (uint32_t)W + 1u;
 #pragma unroll
@@ -370,10 +377,23 @@ struct TileChoice {
   int th, tw, T;
 };
 
+int ext_width(int W, const TileChoice &c) { return c.tw >= W ? W : std::min(W, c.tw + 4 * c.T); }
+
 int ext_cells(int H, int W, const TileChoice &c) {
   const int eh = c.th >= H ? H : std::min(H, c.th + 4 * c.T);
-  const int ew = c.tw >= W ? W : std::min(W, c.tw + 4 * c.T);
-  return eh * ew;
+  return eh * ext_width(W, c);
+}
+
+// LDS floats for the largest extended region ehw (widest ew) and sponge width pad.
+int lds_floats(int ehw, int ew, int pad, int dir, int *plane1, int *tab_off, int *il_off) {
+  const int guard = 2 * ew + 2;
+  const int p1 = guard + ehw + guard;
+  const int tab = (p1 + ehw + guard + 1) & ~1;
+  const int il = tab + 2 * (pad + 1) * (pad + 1);
+  if (plane1) *plane1 = p1;
+  if (tab_off) *tab_off = tab;
+  if (il_off) *il_off = il;
+  return il + (dir ? ehw : 0);
 }
 
 // Cells loaded per launch over all tiles, divided by H*W (halo redundancy).
@@ -393,7 +413,7 @@ double redundancy(int H, int W, const TileChoice &c) {
 // Candidate tilings of an H x W grid that fit a workgroup, best modelled first.
 // Model: per step, every loaded cell is updated once, and each launch pays
 // about two extra updates' worth of time for state loads and stores.
-std::vector<TileChoice> candidates(int H, int W, int dir) {
+std::vector<TileChoice> candidates(int H, int W, int pad, int dir) {
   const int cap = (dir ? kAdjMaxK : kFwdMaxK) * kThreads;
   std::vector<std::pair<double, TileChoice>> c;
   for (int T : {1, 2, 3, 4, 6, 8, 12}) {
@@ -404,7 +424,9 @@ std::vector<TileChoice> candidates(int H, int W, int dir) {
         const int tw = (W + ntx - 1) / ntx;
         if (ntx > 1 && tw < 16) break;
         const TileChoice tc{th, tw, T};
-        if (ext_cells(H, W, tc) > cap) continue;
+        if (ext_cells(H, W, tc) > cap ||
+            lds_floats(ext_cells(H, W, tc), ext_width(W, tc), pad, dir, nullptr, nullptr, nullptr) > kMaxLdsFloats)
+          continue;
         c.push_back({redundancy(H, W, tc) * (1.0 + 2.0 / T), tc});
       }
     }
@@ -452,7 +474,7 @@ int fill_group(const ss_sweep_group &in, int dir, SGroup *o, char *err, size_t e
     if (it != g_tuned.end()) {
       tc = it->second;
     } else {
-      const auto c = candidates(g.H, g.W, dir);
+      const auto c = candidates(g.H, g.W, g.pad, dir);
       if (c.empty()) {
         snprintf(err, errn, "sweep: no tiling of %dx%d fits a workgroup", g.H, g.W);
         return SS_EUNSUP;
@@ -463,7 +485,9 @@ int fill_group(const ss_sweep_group &in, int dir, SGroup *o, char *err, size_t e
   tc.th = std::min(tc.th, g.H);
   tc.tw = std::min(tc.tw, g.W);
   const int cap = (dir ? kAdjMaxK : kFwdMaxK) * kThreads;
-  if (ext_cells(g.H, g.W, tc) > cap) {
+  if (ext_cells(g.H, g.W, tc) > cap ||
+      lds_floats(ext_cells(g.H, g.W, tc), ext_width(g.W, tc), g.pad, dir, nullptr, nullptr, nullptr) >
+          kMaxLdsFloats) {
     snprintf(err, errn, "sweep: tile %dx%d at depth %d needs %d cells per workgroup (max %d)", tc.th, tc.tw, tc.T,
              ext_cells(g.H, g.W, tc), cap);
     return SS_EUNSUP;
@@ -530,7 +554,7 @@ int make_plan(int ngroups, const ss_sweep_group *groups, int dir, Plan *p) {
     item0 += gs[i].nitems;
     p->args.g[i] = gs[i];
     ehw = std::max(ehw, ext_cells(gs[i].H, gs[i].W, {gs[i].th, gs[i].tw, gs[i].T}));
-    ew = std::max(ew, gs[i].tw >= gs[i].W ? gs[i].W : std::min(gs[i].W, gs[i].tw + 4 * gs[i].T));
+    ew = std::max(ew, ext_width(gs[i].W, {gs[i].th, gs[i].tw, gs[i].T}));
     pad = std::max(pad, gs[i].pad);
   }
   if (item0 > (1 << 30)) {
@@ -544,15 +568,19 @@ int make_plan(int ngroups, const ss_sweep_group *groups, int dir, Plan *p) {
   for (int k = 0; k < p->nlaunch; ++k)
     for (int i = 0; i < ngroups; ++i)
       if (gs[i].nlaunch > k) p->nactive[k] = gs[i].item0 + gs[i].nitems;
-  const int guard = 2 * ew + 2;
-  const int tab_off = (2 * guard + ehw + 1) & ~1;
-  const int il_off = tab_off + 2 * (pad + 1) * (pad + 1);
-  p->args.guard = guard;
+  int plane1, tab_off, il_off;
+  const int nlds = lds_floats(ehw, ew, pad, dir, &plane1, &tab_off, &il_off);
+  if (nlds > kMaxLdsFloats) {
+    set_error("sweep: the groups' tilings together need more than 160 KiB of LDS");
+    return SS_EUNSUP;
+  }
+  p->args.plane0 = 2 * ew + 2;
+  p->args.plane1 = plane1;
   p->args.tab_off = tab_off;
   p->args.il_off = il_off;
-  p->lds = (size_t)(il_off + (dir ? ehw : 0)) * sizeof(float);
+  p->lds = (size_t)nlds * sizeof(float);
   const int kneed = (ehw + kThreads - 1) / kThreads;
-  p->K = kneed <= 8 ? 8 : kneed <= 16 ? 16 : kneed <= 24 ? 24 : 32;
+  p->K = kneed <= 4 ? 4 : kneed <= 8 ? 8 : kneed <= 12 ? 12 : 16;
   return SS_OK;
 }
 
@@ -561,13 +589,13 @@ using KernelFn = void (*)(SArgs);
 KernelFn kernel_of(const Plan &p) {
   if (p.dir == 0) {
     switch (p.K) {
+      case 4: return sweep_fwd_kernel<4>;
       case 8: return sweep_fwd_kernel<8>;
-      case 16: return sweep_fwd_kernel<16>;
-      case 24: return sweep_fwd_kernel<24>;
-      default: return sweep_fwd_kernel<32>;
+      case 12: return sweep_fwd_kernel<12>;
+      default: return sweep_fwd_kernel<16>;
     }
   }
-  return p.K <= 8 ? sweep_adj_kernel<8> : sweep_adj_kernel<16>;
+  return p.K <= 4 ? sweep_adj_kernel<4> : p.K <= 8 ? sweep_adj_kernel<8> : sweep_adj_kernel<12>;
 }
 
 int prepare_kernel(const Plan &p, KernelFn *fn) {
@@ -711,7 +739,7 @@ int sweep_autotune(int ngroups, ss_sweep_group *groups, int dir, int max_trials,
     Geom g;
     rc = check_geom(&groups[i].geom, &g);
     if (rc != SS_OK) break;
-    const auto cand = candidates(g.H, g.W, dir);
+    const auto cand = candidates(g.H, g.W, g.pad, dir);
     double best = -1.0;
     TileChoice pick{0, 0, 0};
     const int ntry = std::min<int>((int)cand.size(), max_trials > 0 ? max_trials : 12);
