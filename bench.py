@@ -31,6 +31,18 @@ PEAK_HBM_GBS = 8000.0
 BYTES_PER_CELL_UPDATE = 16          # read p(t-1), p(t), c2; write p(t+1); FP32
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of a kernel of the batched workload, from the two
+    rocprofv3 counter passes (FETCH_SIZE, WRITE_SIZE) scaled by a 1 GiB copy,
+    recorded in profiles/hbm_traffic.json; None when not recorded."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'profiles', 'hbm_traffic.json')
+    try:
+        with open(path) as f:
+            return json.load(f)['kernels'][kernel]['bytes']
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def workload(n_models, seed, sim):
     """Models, observed gathers (device), backgrounds and mute velocities."""
     from shotsweep_amd import background as BG, synth
@@ -170,6 +182,18 @@ def run_mixed(args):
     print(json.dumps(result), flush=True)
 
 
+def copy_probe(nbytes=1 << 30, reps=3):
+    """Plain device copies of known size (read nbytes, write nbytes each): the
+    reference point that scales the FETCH_SIZE / WRITE_SIZE counters to bytes."""
+    x = torch.ones(nbytes // 4, dtype=torch.float32, device='cuda')
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    for _ in range(reps):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({'copy_probe_bytes': nbytes, 'reps': reps}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -184,7 +208,10 @@ def main():
     ap.add_argument('--workload', default='batched', choices=('batched', 'mixed'),
                     help='batched: 64 models of 70x70 (the headline metric); mixed: the uneven batch')
     ap.add_argument('--models-per-size', type=int, default=32)
+    ap.add_argument('--copy-probe', action='store_true', help='only run 1 GiB device copies (counter scaling)')
     args = ap.parse_args()
+    if args.copy_probe:
+        return copy_probe()
     if args.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) != 1:
         print('bench.py measures one GPU: the path shards by model with no exchange in the time loop; '
               'multi-GPU runs are not part of this build', file=sys.stderr)
@@ -235,7 +262,8 @@ def main():
                    'models': args.models, 'shot_images': shot_images, 'seed': args.seed,
                    'history_segment': sim._segment(nb)},
         'roofline': {'bound': 'hbm', 'kernel': dominant, 'achieved': achieved, 'peak': PEAK_HBM_GBS, 'unit': 'GB/s',
-                     'frac': achieved / PEAK_HBM_GBS, 'traffic': None,
+                     'frac': achieved / PEAK_HBM_GBS, 'traffic': measured_traffic(dominant),
+                     'traffic_source': 'profiles/hbm_traffic.json (rocprofv3 FETCH_SIZE x2.0, WRITE_SIZE x1.0)',
                      'kernel_ms': avg, 'algorithmic_bytes_per_launch': alg_bytes},
     }
     if not args.no_cpu:
