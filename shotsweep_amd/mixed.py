@@ -48,6 +48,7 @@ class MixedSweep:
         # per group and direction: (tile_h, tile_w, depth); None = tune or model
         self.tiles = [dict(tiles[k]) if tiles is not None else {} for k in range(len(acquisitions))]
         self._buf = [{} for _ in acquisitions]
+        self._plans = {}
         self.events = None   # list -> (name, start, end) HIP events around the two sweep launches
 
     # ------------------------------------------------------------- buffers
@@ -61,6 +62,7 @@ class MixedSweep:
         return t[:n].view(*shape)
 
     def release(self):
+        self._plans.clear()
         for b in self._buf:
             b.clear()
         for s in self.sims:
@@ -175,12 +177,22 @@ class MixedSweep:
             out[k] = (grad, bufs['illum'], bufs['sim'])
         return out
 
+    def held_bytes(self):
+        """Device bytes of the buffers this sweep keeps between calls."""
+        return sum(t.numel() * t.element_size() for b in self._buf for t in b.values())
+
     def _chunks(self, rows, repeats):
-        """Split each group's rows into the same number of chunks (whole obs rows each)."""
+        """Split each group's rows into the same number of chunks (whole obs rows each).
+
+        The plan is fixed per batch shape on first use, so later calls reuse the
+        same buffers (and captured graphs)."""
+        key = (tuple(rows), tuple(repeats))
+        if key in self._plans:
+            return self._plans[key]
         budget = self.max_history_bytes
         if budget is None:
             free, _ = torch.cuda.mem_get_info(self.device)
-            budget = 0.6 * free
+            budget = 0.6 * (free + self.held_bytes())
         n = max(1, math.ceil(self.history_bytes(rows) / budget))
         while True:
             spans = []
@@ -190,6 +202,7 @@ class MixedSweep:
                 spans.append([(c * per * rep, min(nb, (c + 1) * per * rep)) for c in range(n)])
             worst = max(self.history_bytes([s[c][1] - s[c][0] for s in spans]) for c in range(n))
             if worst <= budget or n >= max(r // p for r, p in zip(rows, repeats) if r):
+                self._plans[key] = (n, spans)
                 return n, spans
             n += 1
 
