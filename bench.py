@@ -266,6 +266,14 @@ def main():
                      'traffic_source': 'profiles/hbm_traffic.json (rocprofv3 FETCH_SIZE x2.0, WRITE_SIZE x1.0)',
                      'kernel_ms': avg, 'algorithmic_bytes_per_launch': alg_bytes},
     }
+    # the drop-in call with host buffers (NumPy in, NumPy out): PCIe-inclusive, never `value`
+    obs_rep = np.repeat(obs_host, 3, 0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sim.images(bgs, obs_rep, v_mute=v_mute)
+    host_s = time.perf_counter() - t0
+    result['host_buffers'] = {'call': 'Sim.images(v_bg, obs, v_mute) with NumPy inputs and outputs',
+                              'seconds': host_s, 'shot_images_per_s': shot_images / host_s}
     if not args.no_cpu:
         rows = 3 * args.cpu_models             # cpu_models models x 3 backgrounds x 5 shots
         base, cimg, cillum = cpu_baseline(bgs, obs_host, v_mute, rows)

@@ -199,3 +199,72 @@ def test_born_matches_finite_differences():
     eps = 1e-3
     num = (sim.gathers(v + eps * dv) - sim.gathers(v - eps * dv)) / (2 * eps)
     assert rel(sim.born(v, dv), num) < 1e-7
+
+
+# ------------------------------------------------------- CNN oracle pins
+def _brute_conv3x3(x, w, b):
+    """Same-padded 3x3 cross-correlation by explicit loops (tiny sizes)."""
+    N, C, H, W = x.shape
+    out = np.zeros((N, w.shape[0], H, W))
+    xp = np.pad(x, ((0, 0), (0, 0), (1, 1), (1, 1)))
+    for o in range(w.shape[0]):
+        acc = np.full((N, H, W), b[o])
+        for c in range(C):
+            for dy in range(3):
+                for dx in range(3):
+                    acc += w[o, c, dy, dx] * xp[:, c, dy:dy + H, dx:dx + W]
+        out[:, o] = acc
+    return out
+
+
+def test_cnn_oracle_forward_is_cross_correlation():
+    from oracle import cnn as ocnn
+    layers = ((2, 3, True), (3, 1, False))
+    rng = np.random.default_rng(0)
+    n = sum(ci * co * 9 + co for ci, co, _ in layers)
+    flat = torch.tensor(rng.standard_normal(n))
+    x = rng.standard_normal((2, 2, 5, 6))
+    (w1, b1, _), (w2, b2, _) = ocnn.unflatten(flat, layers)
+    h = np.maximum(_brute_conv3x3(x, w1.numpy(), b1.numpy()), 0)
+    want = _brute_conv3x3(h, w2.numpy(), b2.numpy())
+    got = ocnn.forward(flat, torch.tensor(x), layers).numpy()
+    np.testing.assert_allclose(got, want, rtol=1e-12, atol=1e-12)
+    # a flipped (true convolution) kernel would differ
+    flipped = _brute_conv3x3(x, w1.numpy()[:, :, ::-1, ::-1], b1.numpy())
+    assert np.abs(flipped - _brute_conv3x3(x, w1.numpy(), b1.numpy())).max() > 1e-3
+
+
+def test_cnn_oracle_gradient_finite_differences():
+    from oracle import cnn as ocnn
+    layers = ((2, 4, True), (4, 4, True), (4, 1, False))
+    n = sum(ci * co * 9 + co for ci, co, _ in layers)
+    g = torch.Generator().manual_seed(1)
+    flat = torch.randn(n, generator=g, dtype=torch.float64) * 0.5
+    x = torch.randn(2, 2, 7, 6, generator=g, dtype=torch.float64)
+    t = torch.randn(2, 1, 7, 6, generator=g, dtype=torch.float64)
+    _, grad = ocnn.loss_and_grad(flat, x, t, layers)
+    idx = torch.randperm(n, generator=g)[:25]
+    h = 1e-6
+    for i in idx.tolist():
+        e = torch.zeros(n, dtype=torch.float64)
+        e[i] = h
+        lp, _ = ocnn.loss_and_grad(flat + e, x, t, layers)
+        lm, _ = ocnn.loss_and_grad(flat - e, x, t, layers)
+        fd_ = (lp - lm) / (2 * h)
+        assert abs(fd_ - float(grad[i])) <= 1e-6 * max(1.0, abs(fd_)), (i, fd_, float(grad[i]))
+
+
+def test_cnn_oracle_adam_closed_form():
+    from oracle import cnn as ocnn
+    p = torch.tensor([1.0, -2.0, 0.5], dtype=torch.float64)
+    g1 = torch.tensor([0.1, -0.3, 0.0], dtype=torch.float64)
+    m = v = torch.zeros(3, dtype=torch.float64)
+    p1, m1, v1 = ocnn.adam(p, g1, m, v, 1, lr=0.01)
+    # step 1: bias-corrected m/sqrt(v) = sign(g) (for g != 0)
+    np.testing.assert_allclose(p1.numpy(), [1.0 - 0.01 * 0.1 / (0.1 + 1e-8), -2.0 + 0.01 * 0.3 / (0.3 + 1e-8), 0.5],
+                               rtol=0, atol=1e-15)
+    g2 = torch.tensor([0.2, 0.1, -0.4], dtype=torch.float64)
+    p2, _, _ = ocnn.adam(p1, g2, m1, v1, 2, lr=0.01)
+    mh = (0.9 * 0.1 * g1 + 0.1 * g2) / (1 - 0.81)
+    vh = (0.999 * 0.001 * g1 ** 2 + 0.001 * g2 ** 2) / (1 - 0.999 ** 2)
+    np.testing.assert_allclose(p2.numpy(), (p1 - 0.01 * mh / (vh.sqrt() + 1e-8)).numpy(), rtol=0, atol=1e-14)
