@@ -10,9 +10,9 @@
 // HBM for the adjoint (gpu.py:172-175).
 //
 // Adjoint (gpu.py:177-184): LDS holds q = (c2 A m1)/12 (zero on the two-cell
-// border), the illumination accumulator of the model cells, the sponge table
-// and row/column descriptors; each thread keeps m1, m2 and the imaging
-// accumulator of its cells in registers (512 threads, 44 cells each), streams
+// border), the illumination accumulator of the model cells and the sponge
+// table; each thread keeps m1, m2, the imaging accumulator and the packed
+// descriptors of its cells in registers (512 threads, 44 cells each), streams
 // the forward Laplacian from HBM a few cells ahead of use, and re-reads c2
 // from the L2-resident c2 plane when it publishes q.
 //
@@ -20,10 +20,9 @@
 // slot j = i / 1024, so every LDS and HBM access of a wavefront touches 64
 // consecutive cells.  Sponge coefficients A, B (gpu.py:85-90) depend only on
 // the distance (dz, dx) into the sponge, so they come from an LDS table of
-// (pad+1)^2 entries.  In the forward each cell carries a 16-bit descriptor
-// (table index or, for model cells, the illumination index, plus two flags),
-// two per register; the adjoint, short of registers, derives the same from
-// per-row and per-column LDS tables.
+// (pad+1)^2 entries.  Each cell carries a 16-bit descriptor (table index or,
+// for model cells, the illumination index, plus interior and model flags),
+// two per register, decoded with two bit operations per step.
 //
 // The 150 x 150 plane (70 x 70 model, pad 40) is compiled with its shape as
 // constants, so stencil offsets fold into instruction immediates; other shapes
@@ -142,18 +141,7 @@ size_t fwd_lds_bytes(const Geom &g) {
 }
 
 size_t adj_lds_bytes(const Geom &g) {
-  return fwd_lds_bytes(g) + (size_t)g.nz * g.nx * sizeof(float) + (size_t)(g.H + g.W) * sizeof(uint32_t);
-}
-
-// Row / column descriptor of the adjoint: bits 0-7 distance into the sponge,
-// bit 8 inside the Laplacian interior [2, n-3], bit 9 inside the model
-// [pad, pad+n_model), bits 16-31 offset from the model's first row/column.
-constexpr uint32_t kLineInterior = 1u << 8, kLineModel = 1u << 9;
-__device__ __forceinline__ uint32_t line_desc(int i, int n, int nmodel, int pad) {
-  uint32_t v = (uint32_t)sponge_dist(i, n, pad);
-  if (i >= 2 && i < n - 2) v |= kLineInterior;
-  if (i >= pad && i < pad + nmodel) v |= kLineModel | ((uint32_t)(i - pad) << 16);
-  return v;
+  return fwd_lds_bytes(g) + (size_t)g.nz * g.nx * sizeof(float);
 }
 
 // NZ, NX, PAD > 0 and ROW >= 0 fix the geometry at compile time (0 / -1: runtime).
@@ -300,8 +288,6 @@ __global__ __launch_bounds__(kAdjThreads) void adj_resident_kernel(Geom g, int t
   float *Q = lds + guard;
   float2 *tab = reinterpret_cast<float2 *>(lds + table_offset(H, W));
   float *il = reinterpret_cast<float *>(tab + (pad + 1) * (pad + 1));
-  uint32_t *zinfo = reinterpret_cast<uint32_t *>(il + nz * nx);
-  uint32_t *xinfo = zinfo + H;
   const int64_t prop = xcd_prop(g);
   if (prop < 0) return;
   const int b = (int)(prop / g.ns), s = (int)(prop - (int64_t)b * g.ns);
@@ -313,24 +299,15 @@ __global__ __launch_bounds__(kAdjThreads) void adj_resident_kernel(Geom g, int t
   }
   fill_table(tab, A, B, W, pad, 1);
   for (int k = t; k < nzx; k += kAdjThreads) il[k] = illumg[prop * nzx + k];
-  for (int k = t; k < H; k += kAdjThreads) zinfo[k] = line_desc(k, H, nz, pad);
-  for (int k = t; k < W; k += kAdjThreads) xinfo[k] = line_desc(k, W, nx, pad);
   __syncthreads();
-  struct Desc { uint32_t tab; bool interior, model; int illum; };
-  // i / W: a constant divisor folds to a multiply; a runtime W uses
-  // ceil(2^32 / W), exact for i * W < 2^32 (planes here hold < 2^16 cells).
-  const uint32_t magic_w = 0xffffffffu / (uint32_t)W + 1u;
-  auto desc = [&](int i) __attribute__((always_inline)) {
-    const int z = NX ? i / W : (int)__umulhi((uint32_t)i, magic_w);
-    const int x = i - z * W;
-    const uint32_t zi = zinfo[z], xi = xinfo[x];
-    Desc d;
-    d.model = (zi & xi & kLineModel) != 0;
-    d.interior = (zi & xi & kLineInterior) != 0;
-    d.tab = d.model ? 0u : (zi & 0xffu) * (uint32_t)(pad + 1) + (xi & 0xffu);
-    d.illum = (int)(zi >> 16) * nx + (int)(xi >> 16);
-    return d;
-  };
+  uint32_t pk[(K + 1) / 2];
+#pragma unroll
+  for (int j = 0; j < (K + 1) / 2; ++j) pk[j] = 0;
+#pragma unroll
+  for (int j = 0; j < K; ++j) {
+    const int i = t + j * kAdjThreads;
+    if (i < HW) pk[j >> 1] |= cell_desc(i, H, W, nz, nx, pad) << ((j & 1) * 16);
+  }
   const __amdgpu_buffer_rsrc_t srs1 = make_rsrc(m1g + prop * HW, HW);
   const __amdgpu_buffer_rsrc_t srs2 = make_rsrc(m2g + prop * HW, HW);
   const __amdgpu_buffer_rsrc_t srsg = make_rsrc(gaccg + prop * HW, HW);
@@ -350,8 +327,8 @@ __global__ __launch_bounds__(kAdjThreads) void adj_resident_kernel(Geom g, int t
   for (int j = 0; j < K; ++j) {
     const int i = t + j * kAdjThreads;
     if (i < HW) {
-      const Desc d = desc(i);
-      Q[i] = d.interior ? div12((buf_load(crs, t * 4, j * kAdjThreads * 4) * tab[d.tab].x) * m1[j]) : 0.0f;
+      const uint32_t d = desc_of(pk, j);
+      Q[i] = (d & kInterior) ? div12((buf_load(crs, t * 4, j * kAdjThreads * 4) * sponge(tab, d).x) * m1[j]) : 0.0f;
     }
   }
   const int rec0 = (pad + row) * W + pad, rec1 = rec0 + nx;
@@ -373,9 +350,10 @@ __global__ __launch_bounds__(kAdjThreads) void adj_resident_kernel(Geom g, int t
     const int slot = it - t0;
     const float wt = w[it];
     int off = t;
+    opaque(pk);
 #pragma unroll
     for (int j = 0; j < K; ++j) {
-      group_fence(j, off);
+      group_fence(j, off, pk);
       const float L = Lb[j % D];
       {
         const int jn = j + D;
@@ -387,10 +365,10 @@ __global__ __launch_bounds__(kAdjThreads) void adj_resident_kernel(Geom g, int t
       }
       if (t + j * kAdjThreads < HW) {
         const int i = off + j * kAdjThreads;
-        const Desc d = desc(i);
+        const uint32_t d = desc_of(pk, j);
         const float lt = lapt_sum(Q[i], Q[i + 2 * W], Q[i + W], Q[i - W], Q[i - 2 * W], Q[i + 2], Q[i + 1],
                                   Q[i - 1], Q[i - 2]);
-        const float2 aab = tab[d.tab];
+        const float2 aab = sponge(tab, d);
         float mu = (2.0f * aab.x) * m1[j];
         mu = mu + lt;
         mu = mu - aab.y * m2[j];
@@ -401,8 +379,8 @@ __global__ __launch_bounds__(kAdjThreads) void adj_resident_kernel(Geom g, int t
         if (j == srci / kAdjThreads) {
           if (i == srci) sg = sg + (mu * wt) * sdv;
         }
-        if (d.model) {
-          float &v = il[d.illum];
+        if (d & kModel) {
+          float &v = il[d & kIdxMask];
           v = v + L * L;
         }
         m2[j] = m1[j];
@@ -413,12 +391,14 @@ __global__ __launch_bounds__(kAdjThreads) void adj_resident_kernel(Geom g, int t
     __syncthreads();
     off = t;
     asm volatile("" : "+v"(off));  // keep per-cell descriptors from being hoisted out of the time loop
+    opaque(pk);
 #pragma unroll
     for (int j = 0; j < K; ++j) {
+      group_fence(j, off, pk);
       if (t + j * kAdjThreads < HW) {
-        const Desc d = desc(off + j * kAdjThreads);
-        const float q = div12((buf_load(crs, off * 4, j * kAdjThreads * 4) * tab[d.tab].x) * m1[j]);
-        Q[off + j * kAdjThreads] = d.interior ? q : 0.0f;
+        const uint32_t d = desc_of(pk, j);
+        const float q = div12((buf_load(crs, off * 4, j * kAdjThreads * 4) * sponge(tab, d).x) * m1[j]);
+        Q[off + j * kAdjThreads] = (d & kInterior) ? q : 0.0f;
       }
     }
     __syncthreads();

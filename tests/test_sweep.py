@@ -50,7 +50,16 @@ def test_sweep_matches_oracle_pipeline():
     diff = np.abs(out['images'].astype(np.float32) - want.astype(np.float32))
     print(f'  raw image rel L2 {e:.3e}; fp16 entries differing: {(diff > 0).mean():.2e}, max {diff.max():.2e}')
     assert e < 1e-5
-    # a ~1e-6 relative image difference flips values that sit on a float16 rounding
-    # boundary by one ulp (2**-10 relative); nothing differs by more than one ulp
-    ulp = np.spacing(np.maximum(np.abs(want.astype(np.float32)), np.float32(2.0 ** -14)).astype(np.float16))
-    assert np.all(diff <= ulp.astype(np.float32)) and (diff > 0).mean() < 0.02
+    # rounding to float16 moves each value by at most half an ulp, so two cache entries
+    # differ by at most one ulp (of the larger) plus the scaled raw difference; near the
+    # zeroed surface rows the Laplacian filter cancels, so a few tiny pixels carry a
+    # larger relative raw difference than the 1e-5 image-wide bound suggests
+    raw = out['raw'].reshape(want.shape).astype(np.float64)
+    ref = img.reshape(want.shape).astype(np.float64)
+    raw[..., :2, :] = 0
+    ref[..., :2, :] = 0
+    scale = np.percentile(np.abs(ref).reshape(2, -1), 99, axis=1)[:, None, None, None, None]
+    big = np.maximum(np.abs(want.astype(np.float32)), np.abs(out['images'].astype(np.float32)))
+    ulp = np.spacing(np.maximum(big, np.float32(2.0 ** -14)).astype(np.float16)).astype(np.float64)
+    bound = ulp + np.abs(raw - ref) / scale * (1 + 1e-3) + big * 1e-5   # last term: the two p99 scales
+    assert np.all(diff <= bound) and (diff > 0).mean() < 0.02
