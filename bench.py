@@ -52,12 +52,12 @@ def workload(n_models, seed, sim):
     obs_host = obs.cpu().numpy()
     trend = BG.depth_trend(models)
     v_s, fallbacks = [], 0
-    for k in range(n_models):
-        try:
-            v_s.append(BG.estimate_surface_velocity(obs_host[k])['velocity'])
-        except ValueError:
+    for k, est in enumerate(BG.estimate_surface_velocity_device(obs)):   # picks on the GPU, fits on the host
+        if isinstance(est, ValueError):
             v_s.append(float(models[k, 0].mean()))
             fallbacks += 1
+        else:
+            v_s.append(est['velocity'])
     v_s = np.asarray(v_s)
     bgs = np.concatenate([BG.backgrounds(v, trend) for v in v_s])
     v_mute = np.repeat(v_s, len(BG.RHOS))

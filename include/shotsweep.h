@@ -223,6 +223,33 @@ int ss_velocity_update_f32(int B, int64_t n, const float *v, const float *d, con
                            float *out, void *stream);
 
 /*
+ * First-break picks for the surface-velocity anchor, replacing the per-trace
+ * loop of background.py:30-40 (amplitude.max over time and the first sample
+ * above frac * peak).  For every trace (gather g, receiver r) of obs
+ * (ngathers, nt, nx): peak[g, r] = max_t |obs|, onset[g, r] = first t with
+ * |obs| > frac * peak (0 if none).  The threshold and comparisons are in double,
+ * so picks on FP32 gathers equal NumPy's on the same data as float64.
+ *   onset (ngathers, nx) int32, peak (ngathers, nx) in the element type
+ */
+int ss_first_breaks_f32(int64_t ngathers, int nt, int nx, const float *obs, double frac, int32_t *onset,
+                        float *peak, void *stream);
+int ss_first_breaks_f64(int64_t ngathers, int nt, int nx, const double *obs, double frac, int32_t *onset,
+                        double *peak, void *stream);
+
+/*
+ * OpenFWI scores of B velocity maps (H x W), replacing physics/metrics.py:14-73
+ * (norm, torch_ssim, metrics): out (B, 3) doubles = MAE and RMSE of the maps
+ * normalised to [-1, 1] by (v - vmin) / (vmax - vmin) * 2 - 1, and SSIM of the
+ * same maps shifted to [0, 1] with the 11 x 11 window (window, 121 doubles, the
+ * outer product of the normalised sigma-1.5 Gaussian), zero padding and
+ * c1 = 0.01^2, c2 = 0.03^2.  Computed in double; sums are tree reductions.
+ */
+int ss_velocity_scores_f32(int B, int H, int W, const float *pred, const float *truth, double vmin, double vmax,
+                           const double *window, double *out, void *stream);
+int ss_velocity_scores_f64(int B, int H, int W, const double *pred, const double *truth, double vmin,
+                           double vmax, const double *window, double *out, void *stream);
+
+/*
  * Mixed-size sweep (FP32).  Up to SS_SWEEP_MAX_GROUPS grid-size groups, each
  * a batch on its own padded grid with its own step count, shot count and
  * source columns, advance together: one tiled, temporally blocked launch per

@@ -11,7 +11,7 @@ import torch
 LIB_PATH = Path(__file__).resolve().parent / 'libshotsweep.so'
 MAX_SHOTS = 32
 IMPL_AUTO, IMPL_STEPWISE, IMPL_RESIDENT = 0, 1, 2
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 _vp = ctypes.c_void_p
 _i = ctypes.c_int
@@ -81,8 +81,15 @@ _SWEEP = {
     'ss_sweep_autotune_f32': [_i, ctypes.POINTER(SSSweepGroup), _i, _i, _vp],
 }
 
+_QC = {
+    'ss_first_breaks_f32': [_i64, _i, _i, _vp, _d, _vp, _vp, _vp],
+    'ss_first_breaks_f64': [_i64, _i, _i, _vp, _d, _vp, _vp, _vp],
+    'ss_velocity_scores_f32': [_i, _i, _i, _vp, _vp, _d, _d, _vp, _vp, _vp],
+    'ss_velocity_scores_f64': [_i, _i, _i, _vp, _vp, _d, _d, _vp, _vp, _vp],
+}
+
 EXPORTS = ['ss_last_error', 'ss_version', 'ss_resident_max_cells'] + [
-    f'ss_{name}_{sfx}' for sfx in ('f32', 'f64') for name in _sig(ctypes.c_float)] + list(_CNN) + list(_SWEEP)
+    f'ss_{name}_{sfx}' for sfx in ('f32', 'f64') for name in _sig(ctypes.c_float)] + list(_CNN) + list(_SWEEP) + list(_QC)
 
 _LIB = None
 
@@ -105,7 +112,7 @@ def load():
             fn = getattr(lib, f'ss_{name}_{sfx}')
             fn.argtypes = args
             fn.restype = ctypes.c_int
-    for name, args in list(_CNN.items()) + list(_SWEEP.items()):
+    for name, args in list(_CNN.items()) + list(_SWEEP.items()) + list(_QC.items()):
         fn_ = getattr(lib, name)
         fn_.argtypes = args
         fn_.restype = ctypes.c_int

@@ -20,17 +20,58 @@ def _fit(offsets, times):
     return slope, float(np.sqrt(np.mean(misfit ** 2)))
 
 
-def estimate_surface_velocity(obs, frac=0.05):
-    """First-break surface velocity with quality diagnostics (background.py:19-83)."""
+def _check(obs, frac):
     obs = np.asarray(obs)
     if obs.shape != (5, NT, NX) or not np.isfinite(obs).all():
         raise ValueError(f'observations must be finite with shape (5, {NT}, {NX})')
     if not 0 < frac < 1:
         raise ValueError('onset threshold frac must lie between zero and one')
-    amplitude = np.abs(obs)
+    return obs
+
+
+def first_breaks(obs, frac=0.05):
+    """Per-trace peak |g| and first sample above frac * peak (background.py:32-40), on the host."""
+    amplitude = np.abs(_check(obs, frac))
     peaks = amplitude.max(axis=1)                                   # (5, NX)
+    return np.argmax(amplitude > frac * peaks[:, None, :], axis=1), peaks
+
+
+def estimate_surface_velocity(obs, frac=0.05):
+    """First-break surface velocity with quality diagnostics (background.py:19-83)."""
+    onset, peaks = first_breaks(obs, frac)
+    return estimate_from_picks(onset, peaks)
+
+
+def estimate_surface_velocity_device(obs, frac=0.05):
+    """Estimates for a batch of device gathers (B, 5, NT, NX): the picks run in a HIP
+    kernel (ss_first_breaks), the fits on the host.  Returns one dict or ValueError
+    per model, in order."""
+    import torch
+    from . import _lib
+    if obs.dim() != 4 or tuple(obs.shape[1:]) != (5, NT, NX) or not bool(torch.isfinite(obs).all()):
+        raise ValueError(f'observations must be finite with shape (5, {NT}, {NX})')
+    if not 0 < frac < 1:
+        raise ValueError('onset threshold frac must lie between zero and one')
+    obs = obs.contiguous()
+    B = obs.shape[0]
+    onset = torch.empty(B, 5, NX, dtype=torch.int32, device=obs.device)
+    peaks = torch.empty(B, 5, NX, dtype=obs.dtype, device=obs.device)
+    _lib.check(_lib.fn('first_breaks', obs.dtype)(B * 5, NT, NX, _lib.ptr(obs), float(frac), _lib.ptr(onset),
+                                                 _lib.ptr(peaks), _lib.stream_handle(obs.device)), 'first_breaks')
+    onset, peaks = onset.cpu().numpy(), peaks.cpu().numpy().astype(np.float64)
+    out = []
+    for k in range(B):
+        try:
+            out.append(estimate_from_picks(onset[k], peaks[k]))
+        except ValueError as error:
+            out.append(error)
+    return out
+
+
+def estimate_from_picks(onset_index, peaks):
+    """The fit and quality checks of background.py:41-83 from per-trace picks (5, NX)."""
     h = np.abs(np.arange(NX)[None, :] * DX - np.asarray(SRC_X)[:, None])
-    onset = np.argmax(amplitude > frac * peaks[:, None, :], axis=1) * DT
+    onset = np.asarray(onset_index) * DT
     use = (h >= 10) & (h <= 250) & (peaks > 0)
     shots, recv = np.nonzero(use)
     offsets, arrivals = h[shots, recv], onset[shots, recv]

@@ -253,13 +253,43 @@ int fold_impl(int64_t n, int nz, int nx, int pad, const T *g, T *out, void *stre
 }  // namespace
 }  // namespace ss
 
+namespace ss {
+namespace {
+
+template <typename T>
+int first_breaks_impl(int64_t ngathers, int nt, int nx, const T *obs, double frac, int32_t *onset, T *peak,
+                      void *stream) {
+  if (ngathers < 1 || nt < 1 || nx < 1 || !obs || !onset || !peak) {
+    set_error("first_breaks: invalid arguments");
+    return SS_EINVAL;
+  }
+  if (!(frac > 0.0 && frac < 1.0)) {
+    set_error("onset threshold frac must lie between zero and one");
+    return SS_EINVAL;
+  }
+  return first_breaks<T>(ngathers, nt, nx, obs, frac, onset, peak, (hipStream_t)stream);
+}
+
+template <typename T>
+int scores_impl(int B, int H, int W, const T *pred, const T *truth, double vmin, double vmax, const double *window,
+                double *out, void *stream) {
+  if (B < 1 || H < 1 || W < 1 || !pred || !truth || !window || !out || !(vmax > vmin)) {
+    set_error("velocity_scores: invalid arguments");
+    return SS_EINVAL;
+  }
+  return velocity_scores<T>(B, H, W, pred, truth, vmin, vmax, window, out, (hipStream_t)stream);
+}
+
+}  // namespace
+}  // namespace ss
+
 using namespace ss;
 
 extern "C" {
 
 const char *ss_last_error(void) { return ss::g_last_error.c_str(); }
 
-int ss_version(void) { return 2; }
+int ss_version(void) { return 3; }
 
 int64_t ss_resident_max_cells(void) { return resident_max_cells(); }
 
@@ -441,6 +471,26 @@ int ss_velocity_update_f32(int B, int64_t n, const float *v, const float *d, con
     return SS_EINVAL;
   }
   return velocity_update(B, n, v, d, alpha, lo, hi, out, (hipStream_t)stream);
+}
+
+int ss_first_breaks_f32(int64_t ngathers, int nt, int nx, const float *obs, double frac, int32_t *onset,
+                        float *peak, void *stream) {
+  return first_breaks_impl<float>(ngathers, nt, nx, obs, frac, onset, peak, stream);
+}
+
+int ss_first_breaks_f64(int64_t ngathers, int nt, int nx, const double *obs, double frac, int32_t *onset,
+                        double *peak, void *stream) {
+  return first_breaks_impl<double>(ngathers, nt, nx, obs, frac, onset, peak, stream);
+}
+
+int ss_velocity_scores_f32(int B, int H, int W, const float *pred, const float *truth, double vmin, double vmax,
+                           const double *window, double *out, void *stream) {
+  return scores_impl<float>(B, H, W, pred, truth, vmin, vmax, window, out, stream);
+}
+
+int ss_velocity_scores_f64(int B, int H, int W, const double *pred, const double *truth, double vmin,
+                           double vmax, const double *window, double *out, void *stream) {
+  return scores_impl<double>(B, H, W, pred, truth, vmin, vmax, window, out, stream);
 }
 
 int ss_sweep_forward_f32(int ngroups, const ss_sweep_group *groups, int use_graph, void *stream) {

@@ -55,6 +55,16 @@ bool resident_fits(const Geom &g);
 int forward_resident(const Geom &g, int t0, int t1, const float *c2, const float *A, const float *B, const float *w,
                      const float *amp, float *p0, float *p1, float *gathers, float *hist, float *ckpt,
                      int ckpt_every, hipStream_t st);
+// First-break picks (onset index, peak |g|) per trace of (ngathers, nt, nx) gathers.
+template <typename T>
+int first_breaks(int64_t ngathers, int nt, int nx, const T *obs, double frac, int *onset, T *peak, hipStream_t st);
+
+// OpenFWI scores (MAE, RMSE, SSIM) of B velocity maps of H x W cells; window is 11 x 11 doubles.
+template <typename T>
+int velocity_scores(int B, int H, int W, const T *pred, const T *truth, double vmin, double vmax,
+                    const double *window, double *out, hipStream_t st);
+size_t scores_lds_bytes(int H, int W);
+
 // Mixed-size tiled sweep (FP32); dir 0 = forward, 1 = adjoint.
 int sweep_run(int ngroups, const ss_sweep_group *groups, int dir, int use_graph, hipStream_t st);
 int sweep_autotune(int ngroups, ss_sweep_group *groups, int dir, int max_trials, hipStream_t st);

@@ -28,7 +28,7 @@ import time
 import numpy as np
 import torch
 
-from . import _lib, gpu, synth
+from . import _lib, gpu, scores, synth
 from .cnn import PrecondCNN
 
 VMIN, VMAX = 1500.0, 4500.0
@@ -207,6 +207,12 @@ def run(pairs=20000, epochs=8, n_eval=256, iters=10, seed=0, log=print):
     plain_ratio = (hp / hp[:, :1]).mean(0)
     learned_ratio = (hl / hl[:, :1]).mean(0)
     tt = torch.tensor(true, dtype=torch.float32, device=sim.device)
+    ts = torch.tensor(start, dtype=torch.float32, device=sim.device)
+
+    def openfwi(v):   # mean OpenFWI MAE / RMSE / SSIM over the held-out models
+        m = scores.velocity_scores(v.contiguous(), tt).mean(0).tolist()
+        return dict(zip(('mae', 'rmse', 'ssim'), m))
+
     result = {
         'pairs': pairs, 'epochs': epochs, 'eval_models': n_eval, 'iters': iters, 'seed': seed,
         'train_loss': losses, 'time_pairs_s': t_pairs, 'time_train_s': t_train,
@@ -217,6 +223,8 @@ def run(pairs=20000, epochs=8, n_eval=256, iters=10, seed=0, log=print):
         'model_rms_error_start': float(((torch.tensor(start, device=sim.device) - tt) ** 2).mean().sqrt()),
         'model_rms_error_plain': float(((vp - tt) ** 2).mean().sqrt()),
         'model_rms_error_learned': float(((vl - tt) ** 2).mean().sqrt()),
+        'openfwi_scores_start': openfwi(ts), 'openfwi_scores_plain': openfwi(vp),
+        'openfwi_scores_learned': openfwi(vl),
     }
     return result, net
 

@@ -268,3 +268,53 @@ def test_cnn_oracle_adam_closed_form():
     mh = (0.9 * 0.1 * g1 + 0.1 * g2) / (1 - 0.81)
     vh = (0.999 * 0.001 * g1 ** 2 + 0.001 * g2 ** 2) / (1 - 0.999 ** 2)
     np.testing.assert_allclose(p2.numpy(), (p1 - 0.01 * mh / (vh.sqrt() + 1e-8)).numpy(), rtol=0, atol=1e-14)
+
+
+# ---------------------------------------------------- OpenFWI score oracle pins
+def _brute_ssim(a, b, size=11, sigma=1.5):
+    """SSIM from the published formula with explicit loops and zero padding."""
+    H, W = a.shape
+    r = size // 2
+    g = np.array([np.exp(-(k - r) ** 2 / (2 * sigma ** 2)) for k in range(size)])
+    g /= g.sum()
+    tot = 0.0
+    for y in range(H):
+        for x in range(W):
+            m = {'a': 0.0, 'b': 0.0, 'aa': 0.0, 'bb': 0.0, 'ab': 0.0}
+            for dy in range(size):
+                for dx in range(size):
+                    yy, xx = y + dy - r, x + dx - r
+                    if 0 <= yy < H and 0 <= xx < W:
+                        wgt = g[dy] * g[dx]
+                        va, vb = a[yy, xx], b[yy, xx]
+                        m['a'] += wgt * va
+                        m['b'] += wgt * vb
+                        m['aa'] += wgt * va * va
+                        m['bb'] += wgt * vb * vb
+                        m['ab'] += wgt * va * vb
+            sa, sb, sab = m['aa'] - m['a'] ** 2, m['bb'] - m['b'] ** 2, m['ab'] - m['a'] * m['b']
+            c1, c2 = 1e-4, 9e-4
+            tot += ((2 * m['a'] * m['b'] + c1) * (2 * sab + c2)) / ((m['a'] ** 2 + m['b'] ** 2 + c1) * (sa + sb + c2))
+    return tot / (H * W)
+
+
+def test_score_oracle_ssim_matches_brute_force():
+    from oracle import scores
+    rng = np.random.default_rng(3)
+    a = rng.uniform(0, 1, (9, 13))
+    b = np.clip(a + 0.1 * rng.standard_normal((9, 13)), 0, 1)
+    assert abs(scores.ssim(a, b) - _brute_ssim(a, b)) < 1e-13
+    assert abs(scores.ssim(a, a) - 1.0) < 1e-14
+    # a planted wrong sigma is caught
+    assert abs(_brute_ssim(a, b, sigma=2.0) - scores.ssim(a, b)) > 1e-4
+
+
+def test_score_oracle_metrics_closed_forms():
+    from oracle import scores
+    rng = np.random.default_rng(4)
+    v = rng.uniform(1500, 4500, (12, 10))
+    m = scores.metrics(v, v)
+    assert m['mae'] == 0 and m['rmse'] == 0 and abs(m['ssim'] - 1) < 1e-14
+    m = scores.metrics(v + 300.0, v)        # +300 m/s = +0.2 in the [-1, 1] scale
+    assert abs(m['mae'] - 0.2) < 1e-12 and abs(m['rmse'] - 0.2) < 1e-12
+    np.testing.assert_allclose(scores.norm([1500.0, 3000.0, 4500.0]), [-1.0, 0.0, 1.0])
