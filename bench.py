@@ -266,6 +266,21 @@ def main():
                      'traffic_source': 'profiles/hbm_traffic.json (rocprofv3 FETCH_SIZE x2.0, WRITE_SIZE x1.0)',
                      'kernel_ms': avg, 'algorithmic_bytes_per_launch': alg_bytes},
     }
+    # first-break picking for the surface-velocity anchors: HIP kernel vs the host NumPy picks
+    from shotsweep_amd import background as BG
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    BG.estimate_surface_velocity_device(obs)
+    dev_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for k in range(len(obs_host)):
+        try:
+            BG.estimate_surface_velocity(obs_host[k].astype(np.float64))
+        except ValueError:
+            pass
+    result['surface_picking'] = {'models': len(obs_host), 'device_s': dev_s,
+                                 'host_s': time.perf_counter() - t0,
+                                 'note': 'device: ss_first_breaks picks + host fits; host: NumPy picks + fits'}
     # the drop-in call with host buffers (NumPy in, NumPy out): PCIe-inclusive, never `value`
     obs_rep = np.repeat(obs_host, 3, 0)
     torch.cuda.synchronize()

@@ -104,6 +104,24 @@ def calibration_valid(work, expected_fingerprint):
         return False
 
 
+def calibrated(work, provenance, models, gathers, training_models):
+    """Source wavelet and depth trend, from a valid calibration cache or computed and
+    cached (2_process.py:137-161): the wavelet from ``gpu.calibrate`` on the
+    calibration models and their gathers, the trend from the training models.
+    Returns (wavelet, trend, metadata)."""
+    from . import gpu
+    from .background import depth_trend
+    work = Path(work)
+    if not calibration_valid(work, fingerprint(provenance)):
+        wavelet = gpu.calibrate(models, gathers)
+        trend = depth_trend(training_models)
+        write_calibration(work, wavelet, trend, provenance)
+    with open(work / 'calibration_cache.json') as f:
+        meta = json.load(f)
+    return (np.load(work / 'source_wavelet.npy', allow_pickle=False),
+            np.load(work / 'depth_trend.npy', allow_pickle=False), meta)
+
+
 # ------------------------------------------------------------ image cache
 def family_arrays(n, results, nb=len(RHOS), ns=5, nz=70, nx=70):
     """Cache arrays for n models from (indices, sweep_batch result) pairs."""

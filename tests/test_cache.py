@@ -90,3 +90,16 @@ def test_family_cache_passes_the_training_reader(tmp_path):
                 and fp == hashlib.sha256(json.dumps(expected, sort_keys=True,
                                                     separators=(',', ':')).encode()).hexdigest())
         assert inp['images'].shape == (4, 3, 5, 70, 70) and inp['surface_velocity'].shape == (4,)
+
+
+def test_calibrated_reuses_a_valid_cache(tmp_path, monkeypatch):
+    from shotsweep_amd import gpu
+    calls = []
+    monkeypatch.setattr(gpu, 'calibrate', lambda m, g: calls.append(1) or np.cos(np.arange(1000.0)))
+    prov = {'schema': 2, 'calibration_indices': [0, 1]}
+    models = np.full((2, 70, 70), 2000.0)
+    w1, t1, m1 = cache.calibrated(tmp_path, prov, models, None, models)
+    w2, t2, m2 = cache.calibrated(tmp_path, prov, models, None, models)
+    assert len(calls) == 1 and np.array_equal(w1, w2) and np.array_equal(t1, t2) and m1 == m2
+    cache.calibrated(tmp_path, {'schema': 2, 'calibration_indices': [0]}, models, None, models)
+    assert len(calls) == 2
