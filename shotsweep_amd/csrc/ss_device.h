@@ -33,6 +33,21 @@ __device__ __forceinline__ float buf_load(__amdgpu_buffer_rsrc_t r, int voff, in
   return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
 }
 
+// Two consecutive floats (8-byte aligned offsets) as one 64-bit buffer access.
+// The 64-bit builtins move a vector of two 32-bit words (word 0 = first float).
+// The words are packed and unpacked through one uint64 bit cast: reading the
+// vector's elements individually after the load is mis-optimised by this
+// compiler (the load is narrowed to one dword that feeds both elements).
+typedef unsigned int ss_words2 __attribute__((__vector_size__(2 * sizeof(unsigned int))));
+__device__ __forceinline__ void buf_store2(float2 v, __amdgpu_buffer_rsrc_t r, int voff, int soff) {
+  const uint64_t u = (uint64_t)__builtin_bit_cast(uint32_t, v.x) | ((uint64_t)__builtin_bit_cast(uint32_t, v.y) << 32);
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(ss_words2, u), r, voff, soff, 0);
+}
+__device__ __forceinline__ float2 buf_load2(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+  const uint64_t u = __builtin_bit_cast(uint64_t, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+  return make_float2(__builtin_bit_cast(float, (uint32_t)u), __builtin_bit_cast(float, (uint32_t)(u >> 32)));
+}
+
 // floor(n / d) with magic = 0xffffffff / d + 1, exact whenever n * d < 2^32:
 // n * magic / 2^32 exceeds n / d by less than n / 2^32 < 1/d, which is no more
 // than the distance from n / d up to the next integer.

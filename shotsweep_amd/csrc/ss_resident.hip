@@ -268,6 +268,134 @@ __global__ __launch_bounds__(kFwdThreads) void fwd_resident_kernel(Geom g, int t
 }
 
 
+// Paired forward for planes of even width: thread t owns the cell pairs
+// (2q, 2q+1), q = t + 1024 j, so its nine-tap stencils come from seven aligned
+// 8-byte LDS reads per pair (the row taps c-2..c+3 and the pairs two and one
+// rows above and below), and each Laplacian store moves two cells.  Per cell
+// the arithmetic is that of fwd_resident_kernel; every flag the forward needs
+// (interior, model, receiver row) starts on an even column, so a pair never
+// mixes the cases except in its two sponge-table indices.
+template <int KP, int NZ, int NX, int PAD, int ROW>
+__global__ __launch_bounds__(kFwdThreads) void fwd_pair_kernel(Geom g, int t0, int t1, const float *__restrict__ c2,
+                                                               const float *__restrict__ A,
+                                                               const float *__restrict__ B,
+                                                               const float *__restrict__ w,
+                                                               const float *__restrict__ amp, float *__restrict__ p0g,
+                                                               float *__restrict__ p1g, float *__restrict__ gathers,
+                                                               float *__restrict__ hist, float *__restrict__ ckpt,
+                                                               int ckpt_every) {
+  extern __shared__ float lds[];
+  constexpr int nz = NZ, nx = NX, pad = PAD, row = ROW;
+  constexpr int H = nz + 2 * pad, W = nx + 2 * pad;
+  constexpr int HW = H * W, NP = HW / 2, W2 = W / 2;
+  static_assert(W % 2 == 0 && HW % 2 == 0, "pairs need an even plane width");
+  constexpr int guard = 2 * W + 2;
+  float *P = lds + guard;
+  float2 *P2 = reinterpret_cast<float2 *>(P);
+  float2 *tab = reinterpret_cast<float2 *>(lds + table_offset(H, W));
+  const int64_t prop = xcd_prop(g);
+  if (prop < 0) return;  // padding workgroup (whole workgroup exits before any barrier)
+  const int b = (int)(prop / g.ns), s = (int)(prop - (int64_t)b * g.ns);
+  const int t = threadIdx.x;
+
+  for (int k = t; k < guard; k += kFwdThreads) {
+    lds[k] = 0.0f;
+    P[HW + k] = 0.0f;
+  }
+  fill_table(tab, A, B, W, pad, 0);
+
+  float2 pn[KP], cc[KP];
+  uint32_t pk[KP];  // two 16-bit cell descriptors per pair
+  const float2 *p0p = reinterpret_cast<const float2 *>(p0g + prop * HW);
+  const float2 *p1p = reinterpret_cast<const float2 *>(p1g + prop * HW);
+  const float2 *c2b = reinterpret_cast<const float2 *>(c2 + (int64_t)b * HW);
+#pragma unroll
+  for (int j = 0; j < KP; ++j) {
+    const int q = t + j * kFwdThreads;
+    pn[j] = cc[j] = make_float2(0.0f, 0.0f);
+    pk[j] = 0;
+    if (q < NP) {
+      pn[j] = p0p[q];
+      P2[q] = p1p[q];
+      cc[j] = c2b[q];
+      pk[j] = cell_desc(2 * q, H, W, nz, nx, pad) | (cell_desc(2 * q + 1, H, W, nz, nx, pad) << 16);
+    }
+  }
+  constexpr int rec0 = (pad + row) * W + pad;
+  const int srci = rec0 + g.src[s];
+  const bool src_owner = ((srci >> 1) % kFwdThreads) == t;
+  const float ampv = amp[prop];
+  const int nslots = t1 - t0;
+  const int nck = ckpt ? (nslots + ckpt_every - 1) / ckpt_every : 0;
+  const bool rec_thread = gathers && t < nx;
+  const __amdgpu_buffer_rsrc_t hrs = make_rsrc(hist ? hist + prop * nslots * HW : p0g, (int64_t)nslots * HW);
+  const __amdgpu_buffer_rsrc_t grs =
+      make_rsrc(gathers ? gathers + prop * g.nt * nx : p0g, (int64_t)g.nt * nx);
+  const __amdgpu_buffer_rsrc_t crs = make_rsrc(ckpt ? ckpt + prop * nck * 2 * HW : p0g, nck * 2 * HW);
+  __syncthreads();
+
+  for (int it = t0; it < t1; ++it) {
+    const int slot = it - t0;
+    opaque(pk);
+    if (rec_thread && it > t0)  // p(it) after injection is the sample of step it-1
+      buf_store(P[rec0 + t], grs, t * 4, (it - 1) * nx * 4);
+    if (ckpt && slot % ckpt_every == 0) {
+      const int base = (slot / ckpt_every) * 2 * HW;
+#pragma unroll
+      for (int j = 0; j < KP; ++j) {
+        const int q = t + j * kFwdThreads;
+        if (q < NP) {
+          buf_store2(pn[j], crs, t * 8, (base + 2 * j * kFwdThreads) * 4);
+          buf_store2(P2[q], crs, t * 8, (base + HW + 2 * j * kFwdThreads) * 4);
+        }
+      }
+    }
+    int off = t;
+#pragma unroll
+    for (int j = 0; j < KP; ++j) {
+      asm volatile("" : "+v"(off), "+v"(pk[j]));  // one pair at a time
+      if (t + j * kFwdThreads < NP) {
+        const int q = off + j * kFwdThreads;
+        const uint32_t d0 = pk[j] & 0xffffu, d1 = pk[j] >> 16;
+        const float2 r0 = P2[q - 1], r1 = P2[q], r2 = P2[q + 1];
+        const float2 um = P2[q - W2], u2m = P2[q - W], up = P2[q + W2], u2p = P2[q + W];
+        const float s0 = lap_sum(u2m.x, um.x, r1.x, up.x, u2p.x, r0.x, r0.y, r1.y, r2.x);
+        const float s1 = lap_sum(u2m.y, um.y, r1.y, up.y, u2p.y, r0.y, r1.x, r2.x, r2.y);
+        const float L0 = (d0 & kInterior) ? div12(s0) : 0.0f;
+        const float L1 = (d1 & kInterior) ? div12(s1) : 0.0f;
+        const float2 ab0 = sponge(tab, d0), ab1 = sponge(tab, d1);
+        pn[j].x = leapfrog(r1.x, pn[j].x, ab0.y, cc[j].x, L0, ab0.x);
+        pn[j].y = leapfrog(r1.y, pn[j].y, ab1.y, cc[j].y, L1, ab1.x);
+        if (hist) buf_store2(make_float2(L0, L1), hrs, off * 8, (slot * HW + 2 * j * kFwdThreads) * 4);
+        asm volatile("" : "+v"(pn[j].x), "+v"(pn[j].y));  // finish this pair before the next starts
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < KP; ++j) {
+      const int q = t + j * kFwdThreads;
+      if (q < NP) {
+        const float2 c = P2[q];
+        P2[q] = pn[j];
+        pn[j] = c;
+      }
+    }
+    if (src_owner) P[srci] = P[srci] + w[it] * ampv;  // gpu.py:114, added after damping
+    __syncthreads();
+  }
+  if (rec_thread) buf_store(P[rec0 + t], grs, t * 4, (t1 - 1) * nx * 4);
+  float2 *p0o = reinterpret_cast<float2 *>(p0g + prop * HW);
+  float2 *p1o = reinterpret_cast<float2 *>(p1g + prop * HW);
+#pragma unroll
+  for (int j = 0; j < KP; ++j) {
+    const int q = t + j * kFwdThreads;
+    if (q < NP) {
+      p0o[q] = pn[j];
+      p1o[q] = P2[q];
+    }
+  }
+}
+
 // NZ, NX, PAD > 0 and ROW >= 0 fix the geometry at compile time (0 / -1: runtime).
 template <int K, int NZ, int NX, int PAD, int ROW>
 __global__ __launch_bounds__(kAdjThreads) void adj_resident_kernel(Geom g, int t0, int t1, const float *__restrict__ c2,
@@ -415,6 +543,179 @@ __global__ __launch_bounds__(kAdjThreads) void adj_resident_kernel(Geom g, int t
   for (int k = t; k < nzx; k += kAdjThreads) illumg[prop * nzx + k] = il[k];
 }
 
+// Paired adjoint for planes of even width (the 150 x 150 plane): thread t owns
+// the cell pairs (2q, 2q+1), q = t + 512 j.  Seven aligned 8-byte LDS reads per
+// pair give both cells' lap_adjoint taps; the Laplacian history, residual and
+// c2 move two cells per buffer instruction; the illumination of a model pair
+// is one 8-byte read-modify-write.  Per cell the arithmetic and its order are
+// those of adj_resident_kernel.  Interior, model and receiver-row ranges all
+// start on even columns, so both cells of a pair share those flags.
+constexpr int kPairPrefetch = 2;  // history loads issued this many pairs ahead
+template <int KP, int NZ, int NX, int PAD, int ROW>
+__global__ __launch_bounds__(kAdjThreads) void adj_pair_kernel(Geom g, int t0, int t1, const float *__restrict__ c2,
+                                                               const float *__restrict__ A,
+                                                               const float *__restrict__ B,
+                                                               const float *__restrict__ w,
+                                                               const float *__restrict__ sd,
+                                                               const float *__restrict__ res,
+                                                               const float *__restrict__ hist, float *__restrict__ m1g,
+                                                               float *__restrict__ m2g, float *__restrict__ gaccg,
+                                                               float *__restrict__ illumg,
+                                                               float *__restrict__ src_grad) {
+  extern __shared__ float lds[];
+  constexpr int nz = NZ, nx = NX, pad = PAD, row = ROW;
+  constexpr int H = nz + 2 * pad, W = nx + 2 * pad;
+  constexpr int HW = H * W, NP = HW / 2, W2 = W / 2;
+  static_assert(W % 2 == 0 && nx % 2 == 0 && pad % 2 == 0, "pairs need even widths");
+  constexpr int guard = 2 * W + 2;
+  float *Q = lds + guard;
+  float2 *Q2 = reinterpret_cast<float2 *>(Q);
+  float2 *tab = reinterpret_cast<float2 *>(lds + table_offset(H, W));
+  float *il = reinterpret_cast<float *>(tab + (pad + 1) * (pad + 1));
+  float2 *il2 = reinterpret_cast<float2 *>(il);
+  const int64_t prop = xcd_prop(g);
+  if (prop < 0) return;
+  const int b = (int)(prop / g.ns), s = (int)(prop - (int64_t)b * g.ns);
+  const int t = threadIdx.x;
+  constexpr int nzx = nz * nx;
+  for (int k = t; k < guard; k += kAdjThreads) {
+    lds[k] = 0.0f;
+    Q[HW + k] = 0.0f;
+  }
+  fill_table(tab, A, B, W, pad, 1);
+  for (int k = t; k < nzx; k += kAdjThreads) il[k] = illumg[prop * nzx + k];
+  __syncthreads();
+  uint32_t pk[KP];
+#pragma unroll
+  for (int j = 0; j < KP; ++j) {
+    const int q = t + j * kAdjThreads;
+    pk[j] = q < NP ? (cell_desc(2 * q, H, W, nz, nx, pad) | (cell_desc(2 * q + 1, H, W, nz, nx, pad) << 16)) : 0u;
+  }
+  const __amdgpu_buffer_rsrc_t srs1 = make_rsrc(m1g + prop * HW, HW);
+  const __amdgpu_buffer_rsrc_t srs2 = make_rsrc(m2g + prop * HW, HW);
+  const __amdgpu_buffer_rsrc_t srsg = make_rsrc(gaccg + prop * HW, HW);
+  const __amdgpu_buffer_rsrc_t crs = make_rsrc(c2 + (int64_t)b * HW, (int64_t)HW);
+  float2 m1[KP], m2[KP], ga[KP];
+#pragma unroll
+  for (int j = 0; j < KP; ++j) {
+    m1[j] = m2[j] = ga[j] = make_float2(0.0f, 0.0f);
+    if (t + j * kAdjThreads < NP) {
+      m1[j] = buf_load2(srs1, t * 8, j * kAdjThreads * 8);
+      m2[j] = buf_load2(srs2, t * 8, j * kAdjThreads * 8);
+      ga[j] = buf_load2(srsg, t * 8, j * kAdjThreads * 8);
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < KP; ++j) {
+    const int q = t + j * kAdjThreads;
+    if (q < NP) {
+      const uint32_t d0 = pk[j] & 0xffffu, d1 = pk[j] >> 16;
+      const float2 cv = buf_load2(crs, t * 8, j * kAdjThreads * 8);
+      Q2[q] = make_float2((d0 & kInterior) ? div12((cv.x * sponge(tab, d0).x) * m1[j].x) : 0.0f,
+                          (d1 & kInterior) ? div12((cv.y * sponge(tab, d1).x) * m1[j].y) : 0.0f);
+    }
+  }
+  constexpr int rec0 = (pad + row) * W + pad, rec1 = rec0 + nx;
+  const int srci = rec0 + g.src[s];
+  const int srcq = srci >> 1;
+  const float sdv = sd[prop];
+  const bool own_src = (srcq % kAdjThreads) == t;
+  float sg = own_src ? src_grad[prop] : 0.0f;
+  const int nslots = t1 - t0;
+  const __amdgpu_buffer_rsrc_t rrs = make_rsrc(res + prop * g.nt * nx, (int64_t)g.nt * nx);
+  const __amdgpu_buffer_rsrc_t hrs = make_rsrc(hist + prop * nslots * HW, (int64_t)nslots * HW);
+  __syncthreads();
+  constexpr int D = kPairPrefetch;
+  static_assert(KP % D == 0, "pair count must be a multiple of the prefetch distance");
+  float2 Lb[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d)
+    Lb[d] = (t + d * kAdjThreads < NP) ? buf_load2(hrs, t * 8, ((t1 - 1 - t0) * HW + 2 * d * kAdjThreads) * 4)
+                                       : make_float2(0.0f, 0.0f);
+  for (int it = t1 - 1; it >= t0; --it) {
+    const int slot = it - t0;
+    const float wt = w[it];
+    int off = t;
+    opaque(pk);
+#pragma unroll
+    for (int j = 0; j < KP; ++j) {
+      asm volatile("" : "+v"(off), "+v"(pk[j]));  // one pair at a time
+      const float2 L = Lb[j % D];
+      {
+        const int jn = j + D;
+        if (jn < KP) {
+          if (t + jn * kAdjThreads < NP)
+            Lb[j % D] = buf_load2(hrs, off * 8, (slot * HW + 2 * jn * kAdjThreads) * 4);
+        } else if (it > t0 && t + (jn - KP) * kAdjThreads < NP) {
+          Lb[j % D] = buf_load2(hrs, off * 8, ((slot - 1) * HW + 2 * (jn - KP) * kAdjThreads) * 4);
+        }
+      }
+      if (t + j * kAdjThreads < NP) {
+        const int q = off + j * kAdjThreads;
+        const uint32_t d0 = pk[j] & 0xffffu, d1 = pk[j] >> 16;
+        const float2 r0 = Q2[q - 1], r1 = Q2[q], r2 = Q2[q + 1];
+        const float2 um = Q2[q - W2], u2m = Q2[q - W], up = Q2[q + W2], u2p = Q2[q + W];
+        const float lt0 = lapt_sum(r1.x, u2p.x, up.x, um.x, u2m.x, r2.x, r1.y, r0.y, r0.x);
+        const float lt1 = lapt_sum(r1.y, u2p.y, up.y, um.y, u2m.y, r2.y, r2.x, r1.x, r0.y);
+        const float2 aab0 = sponge(tab, d0), aab1 = sponge(tab, d1);
+        float mu0 = (2.0f * aab0.x) * m1[j].x;
+        float mu1 = (2.0f * aab1.x) * m1[j].y;
+        mu0 = mu0 + lt0;
+        mu1 = mu1 + lt1;
+        mu0 = mu0 - aab0.y * m2[j].x;
+        mu1 = mu1 - aab1.y * m2[j].y;
+        if (j * kAdjThreads < rec1 / 2 && (j + 1) * kAdjThreads > rec0 / 2) {
+          if (q >= rec0 / 2 && q < rec1 / 2) {
+            const float2 rv = buf_load2(rrs, (2 * q - rec0) * 4, it * nx * 4);
+            mu0 = mu0 + rv.x;
+            mu1 = mu1 + rv.y;
+          }
+        }
+        ga[j].x = ga[j].x + (aab0.x * mu0) * L.x;
+        ga[j].y = ga[j].y + (aab1.x * mu1) * L.y;
+        if (j == srcq / kAdjThreads) {
+          if (q == srcq) sg = sg + (((srci & 1) ? mu1 : mu0) * wt) * sdv;
+        }
+        if (d0 & kModel) {
+          float2 &v = il2[(d0 & kIdxMask) >> 1];
+          v.x = v.x + L.x * L.x;
+          v.y = v.y + L.y * L.y;
+        }
+        m2[j] = m1[j];
+        m1[j] = make_float2(mu0, mu1);
+        asm volatile("" : "+v"(m1[j].x), "+v"(m1[j].y), "+v"(m2[j].x), "+v"(m2[j].y), "+v"(ga[j].x),
+                     "+v"(ga[j].y));
+      }
+    }
+    __syncthreads();
+    off = t;
+    asm volatile("" : "+v"(off));  // keep per-cell descriptors from being hoisted out of the time loop
+    opaque(pk);
+#pragma unroll
+    for (int j = 0; j < KP; ++j) {
+      asm volatile("" : "+v"(off), "+v"(pk[j]));
+      if (t + j * kAdjThreads < NP) {
+        const uint32_t d0 = pk[j] & 0xffffu, d1 = pk[j] >> 16;
+        const float2 cv = buf_load2(crs, off * 8, j * kAdjThreads * 8);
+        const float q0 = div12((cv.x * sponge(tab, d0).x) * m1[j].x);
+        const float q1 = div12((cv.y * sponge(tab, d1).x) * m1[j].y);
+        Q2[off + j * kAdjThreads] = make_float2((d0 & kInterior) ? q0 : 0.0f, (d1 & kInterior) ? q1 : 0.0f);
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int j = 0; j < KP; ++j) {
+    if (t + j * kAdjThreads < NP) {
+      buf_store2(m1[j], srs1, t * 8, j * kAdjThreads * 8);
+      buf_store2(m2[j], srs2, t * 8, j * kAdjThreads * 8);
+      buf_store2(ga[j], srsg, t * 8, j * kAdjThreads * 8);
+    }
+  }
+  if (own_src) src_grad[prop] = sg;
+  for (int k = t; k < nzx; k += kAdjThreads) illumg[prop * nzx + k] = il[k];
+}
+
 int device_lds_limit() {
   static int limit = -1;
   if (limit < 0) {
@@ -473,7 +774,9 @@ int forward_resident(const Geom &g, int t0, int t1, const float *c2, const float
   const size_t lds = fwd_lds_bytes(g);
   return dispatch_fwd(g, [&](auto kc, auto fixed) -> int {
     constexpr bool F = decltype(fixed)::value;
-    auto kern = fwd_resident_kernel<decltype(kc)::value, F ? 70 : 0, F ? 70 : 0, F ? 40 : 0, F ? 1 : -1>;
+    // the fixed 150 x 150 plane runs the paired kernel (11 pairs per thread)
+    auto kern = F ? fwd_pair_kernel<(150 * 150 / 2 + kFwdThreads - 1) / kFwdThreads, 70, 70, 40, 1>
+                  : fwd_resident_kernel<decltype(kc)::value, F ? 70 : 0, F ? 70 : 0, F ? 40 : 0, F ? 1 : -1>;
     hipError_t e = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return hip_status(e, "forward_resident attribute");
     kern<<<dim3((unsigned)xcd_blocks((int64_t)g.nb * g.ns)), dim3(kFwdThreads), lds, st>>>(g, t0, t1, c2, A, B, w, amp, p0,
@@ -488,7 +791,9 @@ int adjoint_resident(const Geom &g, int t0, int t1, const float *c2, const float
   const size_t lds = adj_lds_bytes(g);
   return dispatch_adj(g, [&](auto kc, auto fixed) -> int {
     constexpr bool F = decltype(fixed)::value;
-    auto kern = adj_resident_kernel<decltype(kc)::value, F ? 70 : 0, F ? 70 : 0, F ? 40 : 0, F ? 1 : -1>;
+    // the fixed 150 x 150 plane runs the paired kernel (22 pairs per thread)
+    auto kern = F ? adj_pair_kernel<(150 * 150 / 2 + kAdjThreads - 1) / kAdjThreads, 70, 70, 40, 1>
+                  : adj_resident_kernel<decltype(kc)::value, F ? 70 : 0, F ? 70 : 0, F ? 40 : 0, F ? 1 : -1>;
     hipError_t e = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return hip_status(e, "adjoint_resident attribute");
     kern<<<dim3((unsigned)xcd_blocks((int64_t)g.nb * g.ns)), dim3(kAdjThreads), lds, st>>>(g, t0, t1, c2, A, B, w, sd, res,
