@@ -86,6 +86,22 @@ __device__ __forceinline__ uint32_t cell_desc(int i, int H, int W, int nz, int n
   return v;
 }
 
+// Cell descriptor of the paired kernels (16 bits): bit 0 Laplacian interior,
+// bit 1 model cell, bits 3-13 the byte offset of the cell's sponge-table entry
+// (index * 8).  Model cells sit at sponge depth 0, i.e. entry 0 (A = B = 1), so
+// a table lookup is one mask and one add, with no select.
+constexpr uint32_t kPInterior = 1u, kPModel = 2u, kPTabMask = 0x3ff8u;
+__device__ __forceinline__ uint32_t pair_cell_desc(int i, int H, int W, int nz, int nx, int pad) {
+  const int z = i / W, x = i - z * W;
+  uint32_t v = (uint32_t)(sponge_dist(z, H, pad) * (pad + 1) + sponge_dist(x, W, pad)) << 3;
+  if (z >= pad && z < pad + nz && x >= pad && x < pad + nx) v |= kPModel;
+  if (z >= 2 && z < H - 2 && x >= 2 && x < W - 2) v |= kPInterior;
+  return v;
+}
+__device__ __forceinline__ float2 sponge_at(const float2 *tab, uint32_t d) {
+  return *reinterpret_cast<const float2 *>(reinterpret_cast<const char *>(tab) + (d & kPTabMask));
+}
+
 // Per-cell descriptors are loop-invariant; left visible, the compiler hoists
 // every value derived from them (table addresses, flag masks) out of the time
 // loop and spills.  An empty asm "redefines" them once per step instead.
@@ -289,6 +305,7 @@ __global__ __launch_bounds__(kFwdThreads) void fwd_pair_kernel(Geom g, int t0, i
   constexpr int H = nz + 2 * pad, W = nx + 2 * pad;
   constexpr int HW = H * W, NP = HW / 2, W2 = W / 2;
   static_assert(W % 2 == 0 && HW % 2 == 0, "pairs need an even plane width");
+  static_assert((pad + 1) * (pad + 1) * 8 <= (int)kPTabMask + 8, "sponge-table offsets must fit the descriptor");
   constexpr int guard = 2 * W + 2;
   float *P = lds + guard;
   float2 *P2 = reinterpret_cast<float2 *>(P);
@@ -318,7 +335,7 @@ __global__ __launch_bounds__(kFwdThreads) void fwd_pair_kernel(Geom g, int t0, i
       pn[j] = p0p[q];
       P2[q] = p1p[q];
       cc[j] = c2b[q];
-      pk[j] = cell_desc(2 * q, H, W, nz, nx, pad) | (cell_desc(2 * q + 1, H, W, nz, nx, pad) << 16);
+      pk[j] = pair_cell_desc(2 * q, H, W, nz, nx, pad) | (pair_cell_desc(2 * q + 1, H, W, nz, nx, pad) << 16);
     }
   }
   constexpr int rec0 = (pad + row) * W + pad;
@@ -361,9 +378,9 @@ __global__ __launch_bounds__(kFwdThreads) void fwd_pair_kernel(Geom g, int t0, i
         const float2 um = P2[q - W2], u2m = P2[q - W], up = P2[q + W2], u2p = P2[q + W];
         const float s0 = lap_sum(u2m.x, um.x, r1.x, up.x, u2p.x, r0.x, r0.y, r1.y, r2.x);
         const float s1 = lap_sum(u2m.y, um.y, r1.y, up.y, u2p.y, r0.y, r1.x, r2.x, r2.y);
-        const float L0 = (d0 & kInterior) ? div12(s0) : 0.0f;
-        const float L1 = (d1 & kInterior) ? div12(s1) : 0.0f;
-        const float2 ab0 = sponge(tab, d0), ab1 = sponge(tab, d1);
+        const float L0 = (d0 & kPInterior) ? div12(s0) : 0.0f;
+        const float L1 = (d1 & kPInterior) ? div12(s1) : 0.0f;
+        const float2 ab0 = sponge_at(tab, d0), ab1 = sponge_at(tab, d1);
         pn[j].x = leapfrog(r1.x, pn[j].x, ab0.y, cc[j].x, L0, ab0.x);
         pn[j].y = leapfrog(r1.y, pn[j].y, ab1.y, cc[j].y, L1, ab1.x);
         if (hist) buf_store2(make_float2(L0, L1), hrs, off * 8, (slot * HW + 2 * j * kFwdThreads) * 4);
@@ -550,7 +567,9 @@ __global__ __launch_bounds__(kAdjThreads) void adj_resident_kernel(Geom g, int t
 // is one 8-byte read-modify-write.  Per cell the arithmetic and its order are
 // those of adj_resident_kernel.  Interior, model and receiver-row ranges all
 // start on even columns, so both cells of a pair share those flags.
-constexpr int kPairPrefetch = 2;  // history loads issued this many pairs ahead
+// history loads are issued this many pairs ahead (a divisor of the pair count)
+template <int KP>
+constexpr int pair_prefetch() { return KP % 2 == 0 ? 2 : (KP % 3 == 0 ? 3 : 1); }
 template <int KP, int NZ, int NX, int PAD, int ROW>
 __global__ __launch_bounds__(kAdjThreads) void adj_pair_kernel(Geom g, int t0, int t1, const float *__restrict__ c2,
                                                                const float *__restrict__ A,
@@ -567,6 +586,7 @@ __global__ __launch_bounds__(kAdjThreads) void adj_pair_kernel(Geom g, int t0, i
   constexpr int H = nz + 2 * pad, W = nx + 2 * pad;
   constexpr int HW = H * W, NP = HW / 2, W2 = W / 2;
   static_assert(W % 2 == 0 && nx % 2 == 0 && pad % 2 == 0, "pairs need even widths");
+  static_assert((pad + 1) * (pad + 1) * 8 <= (int)kPTabMask + 8, "sponge-table offsets must fit the descriptor");
   constexpr int guard = 2 * W + 2;
   float *Q = lds + guard;
   float2 *Q2 = reinterpret_cast<float2 *>(Q);
@@ -589,7 +609,8 @@ __global__ __launch_bounds__(kAdjThreads) void adj_pair_kernel(Geom g, int t0, i
 #pragma unroll
   for (int j = 0; j < KP; ++j) {
     const int q = t + j * kAdjThreads;
-    pk[j] = q < NP ? (cell_desc(2 * q, H, W, nz, nx, pad) | (cell_desc(2 * q + 1, H, W, nz, nx, pad) << 16)) : 0u;
+    pk[j] = q < NP ? (pair_cell_desc(2 * q, H, W, nz, nx, pad) | (pair_cell_desc(2 * q + 1, H, W, nz, nx, pad) << 16))
+                   : 0u;
   }
   const __amdgpu_buffer_rsrc_t srs1 = make_rsrc(m1g + prop * HW, HW);
   const __amdgpu_buffer_rsrc_t srs2 = make_rsrc(m2g + prop * HW, HW);
@@ -611,8 +632,8 @@ __global__ __launch_bounds__(kAdjThreads) void adj_pair_kernel(Geom g, int t0, i
     if (q < NP) {
       const uint32_t d0 = pk[j] & 0xffffu, d1 = pk[j] >> 16;
       const float2 cv = buf_load2(crs, t * 8, j * kAdjThreads * 8);
-      Q2[q] = make_float2((d0 & kInterior) ? div12((cv.x * sponge(tab, d0).x) * m1[j].x) : 0.0f,
-                          (d1 & kInterior) ? div12((cv.y * sponge(tab, d1).x) * m1[j].y) : 0.0f);
+      Q2[q] = make_float2((d0 & kPInterior) ? div12((cv.x * sponge_at(tab, d0).x) * m1[j].x) : 0.0f,
+                          (d1 & kPInterior) ? div12((cv.y * sponge_at(tab, d1).x) * m1[j].y) : 0.0f);
     }
   }
   constexpr int rec0 = (pad + row) * W + pad, rec1 = rec0 + nx;
@@ -625,7 +646,7 @@ __global__ __launch_bounds__(kAdjThreads) void adj_pair_kernel(Geom g, int t0, i
   const __amdgpu_buffer_rsrc_t rrs = make_rsrc(res + prop * g.nt * nx, (int64_t)g.nt * nx);
   const __amdgpu_buffer_rsrc_t hrs = make_rsrc(hist + prop * nslots * HW, (int64_t)nslots * HW);
   __syncthreads();
-  constexpr int D = kPairPrefetch;
+  constexpr int D = pair_prefetch<KP>();
   static_assert(KP % D == 0, "pair count must be a multiple of the prefetch distance");
   float2 Lb[D];
 #pragma unroll
@@ -657,7 +678,7 @@ __global__ __launch_bounds__(kAdjThreads) void adj_pair_kernel(Geom g, int t0, i
         const float2 um = Q2[q - W2], u2m = Q2[q - W], up = Q2[q + W2], u2p = Q2[q + W];
         const float lt0 = lapt_sum(r1.x, u2p.x, up.x, um.x, u2m.x, r2.x, r1.y, r0.y, r0.x);
         const float lt1 = lapt_sum(r1.y, u2p.y, up.y, um.y, u2m.y, r2.y, r2.x, r1.x, r0.y);
-        const float2 aab0 = sponge(tab, d0), aab1 = sponge(tab, d1);
+        const float2 aab0 = sponge_at(tab, d0), aab1 = sponge_at(tab, d1);
         float mu0 = (2.0f * aab0.x) * m1[j].x;
         float mu1 = (2.0f * aab1.x) * m1[j].y;
         mu0 = mu0 + lt0;
@@ -676,8 +697,9 @@ __global__ __launch_bounds__(kAdjThreads) void adj_pair_kernel(Geom g, int t0, i
         if (j == srcq / kAdjThreads) {
           if (q == srcq) sg = sg + (((srci & 1) ? mu1 : mu0) * wt) * sdv;
         }
-        if (d0 & kModel) {
-          float2 &v = il2[(d0 & kIdxMask) >> 1];
+        if (d0 & kPModel) {
+          const int c = 2 * q, z = c / W, x = c - z * W;
+          float2 &v = il2[((z - pad) * nx + (x - pad)) >> 1];
           v.x = v.x + L.x * L.x;
           v.y = v.y + L.y * L.y;
         }
@@ -697,9 +719,9 @@ __global__ __launch_bounds__(kAdjThreads) void adj_pair_kernel(Geom g, int t0, i
       if (t + j * kAdjThreads < NP) {
         const uint32_t d0 = pk[j] & 0xffffu, d1 = pk[j] >> 16;
         const float2 cv = buf_load2(crs, off * 8, j * kAdjThreads * 8);
-        const float q0 = div12((cv.x * sponge(tab, d0).x) * m1[j].x);
-        const float q1 = div12((cv.y * sponge(tab, d1).x) * m1[j].y);
-        Q2[off + j * kAdjThreads] = make_float2((d0 & kInterior) ? q0 : 0.0f, (d1 & kInterior) ? q1 : 0.0f);
+        const float q0 = div12((cv.x * sponge_at(tab, d0).x) * m1[j].x);
+        const float q1 = div12((cv.y * sponge_at(tab, d1).x) * m1[j].y);
+        Q2[off + j * kAdjThreads] = make_float2((d0 & kPInterior) ? q0 : 0.0f, (d1 & kPInterior) ? q1 : 0.0f);
       }
     }
     __syncthreads();
