@@ -362,12 +362,277 @@ __global__ __launch_bounds__(kThreads) void sweep_adj_kernel(SArgs a) {
   }
 }
 
+// Paired variants, used when every group has an even plane width, pad and
+// model width and every tile width is even: thread t owns the cell pairs
+// (2p, 2p+1) of the extended region, p = t + 1024 j.  The nine taps of both
+// cells come from seven aligned 8-byte LDS reads, the history moves two cells
+// per buffer instruction, and interior, own-tile and receiver-row flags are the
+// same for both cells (their ranges start on even columns).  Two descriptor
+// words per pair: A = plane offset of the first cell | flags (one source flag
+// per cell), B = the two cells' sponge-table byte offsets.  Per cell the
+// arithmetic is that of the cell kernels above.
+constexpr uint32_t kQInt = 1u << 17, kQOwn = 1u << 18, kQRow = 1u << 19, kQSrc0 = 1u << 20, kQSrc1 = 1u << 21;
+constexpr uint32_t kTabOff = 0x3ff8u;
+
+__device__ __forceinline__ float2 tab_at(const float2 *tab, uint32_t byte_off) {
+  return *reinterpret_cast<const float2 *>(reinterpret_cast<const char *>(tab) + byte_off);
+}
+
+template <int KP>
+__global__ __launch_bounds__(kThreads) void sweep_fwd_pair_kernel(SArgs a) {
+  extern __shared__ float lds[];
+  const int item = xcd_item();
+  if (item >= a.nactive) return;  // padding workgroup: exits before any barrier
+  const SGroup &G = a.g[find_group(a, item)];
+  const ItemView v = item_view(G, item);
+  const int H = G.H, W = G.W, pad = G.pad, HW = H * W;
+  const int EW = v.EW, EW2 = EW / 2, NPE = v.EH * v.EW / 2;
+  const int t0 = a.k * G.T, n = min(G.T, G.nt - t0);
+  const int64_t PS = (int64_t)(G.nitems / G.ntiles) * HW;
+  const int ib = a.k & 1;
+  const float *p0i = G.st + 2 * ib * PS + (int64_t)v.prop * HW, *p1i = p0i + PS;
+  float *p0o = G.st + 2 * (ib ^ 1) * PS + (int64_t)v.prop * HW, *p1o = p0o + PS;
+  float *Pc = lds + a.plane0, *Pn = lds + a.plane1;
+  float2 *tab = reinterpret_cast<float2 *>(lds + a.tab_off);
+  const int t = threadIdx.x;
+  zero_guards(Pc, Pn, 2 * NPE, EW);
+  sweep_table(tab, G, 0);
+
+  const uint32_t magic = 0xffffffffu / (uint32_t)EW + 1u;
+  const int r = pad + G.row, sxc = pad + G.src[v.s];
+  const float2 *c2b = reinterpret_cast<const float2 *>(G.c2 + (int64_t)v.b * HW);
+  float2 pn[KP], cc[KP];
+  uint32_t da[KP], db[KP];
+#pragma unroll
+  for (int j = 0; j < KP; ++j) {
+    const int p = t + j * kThreads;
+    pn[j] = cc[j] = make_float2(0.0f, 0.0f);
+    da[j] = db[j] = 0;
+    if (p < NPE) {
+      const int e = 2 * p;
+      const int q = (int)magic_div((uint32_t)e, magic);
+      const int z = v.ey0 + q, x = v.ex0 + (e - q * EW);
+      const int gi = z * W + x;
+      uint32_t d = (uint32_t)gi;
+      if (z >= 2 && z < H - 2 && x >= 2 && x < W - 2) d |= kQInt;
+      if (z >= v.y0 && z < v.y1 && x >= v.x0 && x < v.x1) d |= kQOwn;
+      if (z == r && x == sxc) d |= kQSrc0;
+      if (z == r && x + 1 == sxc) d |= kQSrc1;
+      da[j] = d;
+      const int sz = sponge_dist(z, H, pad) * (pad + 1);
+      db[j] = ((uint32_t)(sz + sponge_dist(x, W, pad)) << 3) | ((uint32_t)(sz + sponge_dist(x + 1, W, pad)) << 19);
+      cc[j] = c2b[gi >> 1];
+      float2 *P2 = reinterpret_cast<float2 *>(Pc);
+      if (a.k > 0) {
+        pn[j] = reinterpret_cast<const float2 *>(p0i)[gi >> 1];
+        P2[p] = reinterpret_cast<const float2 *>(p1i)[gi >> 1];
+      } else {
+        P2[p] = make_float2(0.0f, 0.0f);
+      }
+    }
+  }
+  const __amdgpu_buffer_rsrc_t hrs = make_rsrc(G.hist + (int64_t)v.prop * G.nt * HW, (int64_t)G.nt * HW);
+  const int xa = max(v.x0, pad), xb = min(v.x1, pad + G.nx);
+  const bool rec = G.gathers != nullptr && r >= v.y0 && r < v.y1 && t < xb - xa;
+  const int rec_e = (r - v.ey0) * EW + (xa + t - v.ex0);
+  float *grec = G.gathers ? G.gathers + (int64_t)v.prop * G.nt * G.nx + (xa - pad) + t : nullptr;
+  const float ampv = G.amp[v.prop];
+  __syncthreads();
+
+  for (int st = 0; st < n; ++st) {
+    const int it = t0 + st;
+    if (rec && st > 0) grec[(int64_t)(it - 1) * G.nx] = Pc[rec_e];  // p(it) is the sample of step it-1
+    const float wsrc = G.w[it] * ampv;
+    const float2 *C2 = reinterpret_cast<const float2 *>(Pc);
+    float2 *N2 = reinterpret_cast<float2 *>(Pn);
+#pragma unroll
+    for (int j = 0; j < KP; ++j) {
+      const int p = t + j * kThreads;
+      if (p < NPE) {
+        const uint32_t d = da[j];
+        const float2 r0 = C2[p - 1], r1 = C2[p], r2 = C2[p + 1];
+        const float2 um = C2[p - EW2], u2m = C2[p - EW], up = C2[p + EW2], u2p = C2[p + EW];
+        const float s0 = lap_sum(u2m.x, um.x, r1.x, up.x, u2p.x, r0.x, r0.y, r1.y, r2.x);
+        const float s1 = lap_sum(u2m.y, um.y, r1.y, up.y, u2p.y, r0.y, r1.x, r2.x, r2.y);
+        const bool in = (d & kQInt) != 0;
+        const float L0 = in ? div12(s0) : 0.0f;
+        const float L1 = in ? div12(s1) : 0.0f;
+        const float2 ab0 = tab_at(tab, db[j] & kTabOff), ab1 = tab_at(tab, (db[j] >> 16) & kTabOff);
+        float q0 = leapfrog(r1.x, pn[j].x, ab0.y, cc[j].x, L0, ab0.x);
+        float q1 = leapfrog(r1.y, pn[j].y, ab1.y, cc[j].y, L1, ab1.x);
+        const float s0w = q0 + wsrc, s1w = q1 + wsrc;
+        q0 = (d & kQSrc0) ? s0w : q0;
+        q1 = (d & kQSrc1) ? s1w : q1;
+        buf_store2(make_float2(L0, L1), hrs, (d & kQOwn) ? (int)(d & kOffMask) * 4 : kNoAccess, it * HW * 4);
+        N2[p] = make_float2(q0, q1);
+        pn[j] = r1;
+      }
+    }
+    __syncthreads();
+    float *tmp = Pc;
+    Pc = Pn;
+    Pn = tmp;
+  }
+  if (rec) grec[(int64_t)(t0 + n - 1) * G.nx] = Pc[rec_e];
+#pragma unroll
+  for (int j = 0; j < KP; ++j) {
+    const int p = t + j * kThreads;
+    if (p < NPE && (da[j] & kQOwn)) {
+      const int gi = (int)(da[j] & kOffMask);
+      reinterpret_cast<float2 *>(p0o)[gi >> 1] = pn[j];
+      reinterpret_cast<float2 *>(p1o)[gi >> 1] = reinterpret_cast<const float2 *>(Pc)[p];
+    }
+  }
+}
+
+template <int KP>
+__global__ __launch_bounds__(kThreads) void sweep_adj_pair_kernel(SArgs a) {
+  extern __shared__ float lds[];
+  const int item = xcd_item();
+  if (item >= a.nactive) return;
+  const SGroup &G = a.g[find_group(a, item)];
+  const ItemView v = item_view(G, item);
+  const int H = G.H, W = G.W, pad = G.pad, HW = H * W;
+  const int EW = v.EW, EW2 = EW / 2, NPE = v.EH * v.EW / 2;
+  const int hi = G.nt - a.k * G.T, n = min(G.T, hi);
+  const int64_t PS = (int64_t)(G.nitems / G.ntiles) * HW;
+  const int ib = a.k & 1;
+  const float2 *m1i = reinterpret_cast<const float2 *>(G.st + 2 * ib * PS + (int64_t)v.prop * HW);
+  const float2 *m2i = reinterpret_cast<const float2 *>(G.st + (2 * ib + 1) * PS + (int64_t)v.prop * HW);
+  float2 *m1o = reinterpret_cast<float2 *>(G.st + 2 * (ib ^ 1) * PS + (int64_t)v.prop * HW);
+  float2 *m2o = reinterpret_cast<float2 *>(G.st + (2 * (ib ^ 1) + 1) * PS + (int64_t)v.prop * HW);
+  float *Qc = lds + a.plane0, *Qn = lds + a.plane1;
+  float2 *tab = reinterpret_cast<float2 *>(lds + a.tab_off);
+  float2 *il = reinterpret_cast<float2 *>(lds + a.il_off);
+  const int t = threadIdx.x;
+  zero_guards(Qc, Qn, 2 * NPE, EW);
+  sweep_table(tab, G, 1);
+  __syncthreads();
+
+  const uint32_t magic = 0xffffffffu / (uint32_t)EW + 1u;
+  const int r = pad + G.row, sxc = pad + G.src[v.s];
+  const float2 *c2b = reinterpret_cast<const float2 *>(G.c2 + (int64_t)v.b * HW);
+  float2 *gac = reinterpret_cast<float2 *>(G.gacc + (int64_t)v.prop * HW);
+  float2 *ilg = reinterpret_cast<float2 *>(G.illum + (int64_t)v.prop * G.nz * G.nx);
+  float2 m1[KP], m2[KP], ga[KP], ca[KP];
+  uint32_t da[KP], db[KP];
+  float sg = 0.0f;
+#pragma unroll
+  for (int j = 0; j < KP; ++j) {
+    const int p = t + j * kThreads;
+    m1[j] = m2[j] = ga[j] = ca[j] = make_float2(0.0f, 0.0f);
+    da[j] = db[j] = 0;
+    if (p < NPE) {
+      const int e = 2 * p;
+      const int q = (int)magic_div((uint32_t)e, magic);
+      const int z = v.ey0 + q, x = v.ex0 + (e - q * EW);
+      const int gi = z * W + x;
+      uint32_t d = (uint32_t)gi;
+      const bool interior = z >= 2 && z < H - 2 && x >= 2 && x < W - 2;
+      const bool own = z >= v.y0 && z < v.y1 && x >= v.x0 && x < v.x1;
+      const bool model = z >= pad && z < pad + G.nz && x >= pad && x < pad + G.nx;
+      if (interior) d |= kQInt;
+      if (own) d |= kQOwn;
+      if (z == r && x >= pad && x < pad + G.nx) d |= kQRow;
+      if (own && z == r && x == sxc) d |= kQSrc0;
+      if (own && z == r && x + 1 == sxc) d |= kQSrc1;
+      if (d & (kQSrc0 | kQSrc1)) sg = a.k > 0 ? G.sgrad[v.prop] : 0.0f;
+      da[j] = d;
+      const int sz = sponge_dist(z, H, pad) * (pad + 1);
+      const uint32_t o0 = (uint32_t)(sz + sponge_dist(x, W, pad)) << 3;
+      const uint32_t o1 = (uint32_t)(sz + sponge_dist(x + 1, W, pad)) << 3;
+      db[j] = o0 | (o1 << 16);
+      const float2 cv = c2b[gi >> 1];
+      ca[j] = make_float2(cv.x * tab_at(tab, o0).x, cv.y * tab_at(tab, o1).x);  // c2 * A (gpu.py:179)
+      float2 ilv = make_float2(0.0f, 0.0f);
+      if (a.k > 0) {
+        m1[j] = m1i[gi >> 1];
+        m2[j] = m2i[gi >> 1];
+        if (own) ga[j] = gac[gi >> 1];
+        if (own && model) ilv = ilg[((z - pad) * G.nx + (x - pad)) >> 1];
+      }
+      il[p] = ilv;
+      reinterpret_cast<float2 *>(Qc)[p] = make_float2(interior ? div12(ca[j].x * m1[j].x) : 0.0f,
+                                                       interior ? div12(ca[j].y * m1[j].y) : 0.0f);
+    }
+  }
+  const __amdgpu_buffer_rsrc_t hrs = make_rsrc(G.hist + (int64_t)v.prop * G.nt * HW, (int64_t)G.nt * HW);
+  const __amdgpu_buffer_rsrc_t rrs = make_rsrc(G.res + (int64_t)v.prop * G.nt * G.nx, (int64_t)G.nt * G.nx);
+  const int rec0 = r * W + pad;
+  const float sdv = G.amp[v.prop];
+  __syncthreads();
+
+  for (int st = 0; st < n; ++st) {
+    const int it = hi - 1 - st;
+    const float wt = G.w[it];
+    const float2 *C2 = reinterpret_cast<const float2 *>(Qc);
+    float2 *N2 = reinterpret_cast<float2 *>(Qn);
+#pragma unroll
+    for (int j = 0; j < KP; ++j) {
+      const int p = t + j * kThreads;
+      if (p < NPE) {
+        const uint32_t d = da[j];
+        const int gi = (int)(d & kOffMask);
+        const float2 L = buf_load2(hrs, (d & kQOwn) ? gi * 4 : kNoAccess, it * HW * 4);  // 0 off the own tile
+        const float2 r0 = C2[p - 1], r1 = C2[p], r2 = C2[p + 1];
+        const float2 um = C2[p - EW2], u2m = C2[p - EW], up = C2[p + EW2], u2p = C2[p + EW];
+        const float lt0 = lapt_sum(r1.x, u2p.x, up.x, um.x, u2m.x, r2.x, r1.y, r0.y, r0.x);
+        const float lt1 = lapt_sum(r1.y, u2p.y, up.y, um.y, u2m.y, r2.y, r2.x, r1.x, r0.y);
+        const float2 aab0 = tab_at(tab, db[j] & kTabOff), aab1 = tab_at(tab, (db[j] >> 16) & kTabOff);
+        float mu0 = (2.0f * aab0.x) * m1[j].x;
+        float mu1 = (2.0f * aab1.x) * m1[j].y;
+        mu0 = mu0 + lt0;
+        mu1 = mu1 + lt1;
+        mu0 = mu0 - aab0.y * m2[j].x;
+        mu1 = mu1 - aab1.y * m2[j].y;
+        if (d & kQRow) {
+          const float2 rv = buf_load2(rrs, (gi - rec0) * 4, it * G.nx * 4);
+          mu0 = mu0 + rv.x;
+          mu1 = mu1 + rv.y;
+        }
+        const float g0 = ga[j].x + (aab0.x * mu0) * L.x;
+        const float g1 = ga[j].y + (aab1.x * mu1) * L.y;
+        const bool own = (d & kQOwn) != 0;
+        ga[j] = make_float2(own ? g0 : ga[j].x, own ? g1 : ga[j].y);
+        const float2 iv = il[p];
+        il[p] = make_float2(iv.x + L.x * L.x, iv.y + L.y * L.y);  // unchanged off the own tile (L = 0)
+        if (d & kQSrc0) sg = sg + (mu0 * wt) * sdv;
+        if (d & kQSrc1) sg = sg + (mu1 * wt) * sdv;
+        m2[j] = m1[j];
+        m1[j] = make_float2(mu0, mu1);
+        const bool in = (d & kQInt) != 0;
+        N2[p] = make_float2(in ? div12(ca[j].x * mu0) : 0.0f, in ? div12(ca[j].y * mu1) : 0.0f);
+      }
+    }
+    __syncthreads();
+    float *tmp = Qc;
+    Qc = Qn;
+    Qn = tmp;
+  }
+  const uint32_t magic_w = 0xffffffffu / (uint32_t)W + 1u;
+#pragma unroll
+  for (int j = 0; j < KP; ++j) {
+    const int p = t + j * kThreads;
+    const uint32_t d = da[j];
+    if (p < NPE && (d & kQOwn)) {
+      const int gi = (int)(d & kOffMask);
+      m1o[gi >> 1] = m1[j];
+      m2o[gi >> 1] = m2[j];
+      gac[gi >> 1] = ga[j];
+      const int z = (int)__umulhi((uint32_t)gi, magic_w), x = gi - z * W;
+      if (z >= pad && z < pad + G.nz && x >= pad && x < pad + G.nx) ilg[((z - pad) * G.nx + (x - pad)) >> 1] = il[p];
+      if (d & (kQSrc0 | kQSrc1)) G.sgrad[v.prop] = sg;
+    }
+  }
+}
+
 // ------------------------------------------------------------------ planning
 
 struct Plan {
   SArgs args;
   int dir = 0;  // 0 forward, 1 adjoint
-  int K = 0;
+  int K = 0;           // cells per thread (cell kernels) or pairs per thread (paired kernels)
+  bool paired = false;
   size_t lds = 0;
   int nlaunch = 0;
   std::vector<int> nactive;
@@ -421,7 +686,8 @@ std::vector<TileChoice> candidates(int H, int W, int pad, int dir) {
       const int th = (H + nty - 1) / nty;
       if (nty > 1 && th < 8) break;
       for (int ntx = 1; ntx <= 12; ++ntx) {
-        const int tw = (W + ntx - 1) / ntx;
+        int tw = (W + ntx - 1) / ntx;
+        if (W % 2 == 0 && tw < W) tw += tw & 1;  // even widths keep cell pairs inside tiles
         if (ntx > 1 && tw < 16) break;
         const TileChoice tc{th, tw, T};
         if (ext_cells(H, W, tc) > cap ||
@@ -484,6 +750,7 @@ int fill_group(const ss_sweep_group &in, int dir, SGroup *o, char *err, size_t e
   }
   tc.th = std::min(tc.th, g.H);
   tc.tw = std::min(tc.tw, g.W);
+  if (g.W % 2 == 0 && tc.tw < g.W) tc.tw += tc.tw & 1;  // even widths keep cell pairs inside tiles
   const int cap = (dir ? kAdjMaxK : kFwdMaxK) * kThreads;
   if (ext_cells(g.H, g.W, tc) > cap ||
       lds_floats(ext_cells(g.H, g.W, tc), ext_width(g.W, tc), g.pad, dir, nullptr, nullptr, nullptr) >
@@ -579,14 +846,36 @@ int make_plan(int ngroups, const ss_sweep_group *groups, int dir, Plan *p) {
   p->args.tab_off = tab_off;
   p->args.il_off = il_off;
   p->lds = (size_t)nlds * sizeof(float);
-  const int kneed = (ehw + kThreads - 1) / kThreads;
-  p->K = kneed <= 4 ? 4 : kneed <= 8 ? 8 : kneed <= 12 ? 12 : 16;
+  // cell pairs when every plane width, pad and model width is even and every tile width is even or whole
+  p->paired = true;
+  for (int i = 0; i < ngroups; ++i) {
+    const SGroup &G = gs[i];
+    if (G.W % 2 || G.pad % 2 || G.nx % 2 || (G.tw % 2 && G.tw < G.W)) p->paired = false;
+  }
+  if (p->paired) {
+    const int kp = (ehw / 2 + kThreads - 1) / kThreads;
+    p->K = kp <= 2 ? 2 : kp <= 4 ? 4 : kp <= 6 ? 6 : 8;
+  } else {
+    const int kneed = (ehw + kThreads - 1) / kThreads;
+    p->K = kneed <= 4 ? 4 : kneed <= 8 ? 8 : kneed <= 12 ? 12 : 16;
+  }
   return SS_OK;
 }
 
 using KernelFn = void (*)(SArgs);
 
 KernelFn kernel_of(const Plan &p) {
+  if (p.paired) {
+    if (p.dir == 0) {
+      switch (p.K) {
+        case 2: return sweep_fwd_pair_kernel<2>;
+        case 4: return sweep_fwd_pair_kernel<4>;
+        case 6: return sweep_fwd_pair_kernel<6>;
+        default: return sweep_fwd_pair_kernel<8>;
+      }
+    }
+    return p.K <= 2 ? sweep_adj_pair_kernel<2> : p.K <= 4 ? sweep_adj_pair_kernel<4> : sweep_adj_pair_kernel<6>;
+  }
   if (p.dir == 0) {
     switch (p.K) {
       case 4: return sweep_fwd_kernel<4>;

@@ -24,6 +24,9 @@ PAD = 10
 # long enough (0.23-0.26 s) for the residual to survive the 0.12 s direct-wave mute
 ACQS = [Acquisition(nz=20, nx=30, nt=261, ns=3), Acquisition(nz=26, nx=22, nt=235, ns=2),
         Acquisition(nz=16, nx=40, nt=250, ns=4, row=3)]
+# odd plane widths (nx + 2 pad) run the cell kernels; the even ones above the paired kernels
+ACQS_ODD = [Acquisition(nz=20, nx=31, nt=261, ns=3), Acquisition(nz=26, nx=23, nt=235, ns=2),
+            Acquisition(nz=16, nx=41, nt=250, ns=4, row=3)]
 NB = (2, 3, 1)
 TILINGS = [(8, 12, 1), (13, 17, 3), (60, 60, 2), (11, 64, 4), (30, 9, 5)]
 
@@ -78,11 +81,13 @@ def setup_parts(ms, seed=0):
     return parts
 
 
+@pytest.mark.parametrize('layout', ['pairs', 'cells'])
 @pytest.mark.parametrize('tiling', TILINGS)
-def test_sweep_forward_and_adjoint_bit_identical_to_stepwise(tiling):
-    case(f'sweep tiling={tiling}')
-    tiles = [{FORWARD: tiling, ADJOINT: tiling} for _ in ACQS]
-    ms = MixedSweep(ACQS, pad=PAD, autotune=False, use_graph=False, tiles=tiles)
+def test_sweep_forward_and_adjoint_bit_identical_to_stepwise(tiling, layout):
+    case(f'sweep tiling={tiling} {layout}')
+    acqs = ACQS if layout == 'pairs' else ACQS_ODD
+    tiles = [{FORWARD: tiling, ADJOINT: tiling} for _ in acqs]
+    ms = MixedSweep(acqs, pad=PAD, autotune=False, use_graph=False, tiles=tiles)
     parts = setup_parts(ms)
     ms._run(parts, FORWARD)
     gen = torch.Generator(device=DEV).manual_seed(3)
