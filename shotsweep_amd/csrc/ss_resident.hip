@@ -16,17 +16,20 @@
 // the forward Laplacian from HBM a few cells ahead of use, and re-reads c2
 // from the L2-resident c2 plane when it publishes q.
 //
-// Cell ownership: cell i of the H*W plane belongs to thread i % 1024 as its
-// slot j = i / 1024, so every LDS and HBM access of a wavefront touches 64
+// Cell ownership: cell i of the H*W plane belongs to thread i % threads as its
+// slot j = i / threads, so every LDS and HBM access of a wavefront touches 64
 // consecutive cells.  Sponge coefficients A, B (gpu.py:85-90) depend only on
 // the distance (dz, dx) into the sponge, so they come from an LDS table of
 // (pad+1)^2 entries.  Each cell carries a 16-bit descriptor (table index or,
 // for model cells, the illumination index, plus interior and model flags),
 // two per register, decoded with two bit operations per step.
 //
-// The 150 x 150 plane (70 x 70 model, pad 40) is compiled with its shape as
-// constants, so stencil offsets fold into instruction immediates; other shapes
-// use the same code with runtime dimensions.
+// The 150 x 150 plane (70 x 70 model, pad 40) runs paired kernels compiled
+// with its shape as constants (fwd_pair_kernel, adj_pair_kernel): a thread
+// owns cell pairs (2q, 2q+1), reads both cells' taps with 8-byte LDS accesses
+// and moves history, c2 and residual two cells per buffer instruction; their
+// descriptors hold the sponge-table byte offset directly.  Other shapes run
+// the cell kernels with runtime dimensions.
 #include <algorithm>
 
 #include "ss_device.h"

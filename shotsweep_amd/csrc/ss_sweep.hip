@@ -21,9 +21,11 @@
 // replayed.  Tile shape and depth per group come from the caller, from the
 // autotuner (trial launches timed with HIP events), or from a cost model.
 //
-// Arithmetic per cell is that of the stepwise kernels (ss_common.h), so the
-// results are bit-identical to them: forward gpu.py:111-115 and 126-135,
-// adjoint gpu.py:177-184.
+// When all plane widths, pads and model widths are even, paired variants run
+// (two adjacent cells per thread slot, 8-byte LDS and buffer accesses); the
+// cell kernels handle every other layout.  Arithmetic per cell is that of the
+// stepwise kernels (ss_common.h), so the results are bit-identical to them:
+// forward gpu.py:111-115 and 126-135, adjoint gpu.py:177-184.
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
