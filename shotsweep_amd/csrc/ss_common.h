@@ -91,6 +91,25 @@ __host__ __device__ __forceinline__ float div12(float s) {
   return __builtin_copysignf(__builtin_fmaf(r, R, q0), s);
 }
 
+// div12 of two values with one branch: both take the fast path unless either
+// needs IEEE division, and then each falls back only if it needs to.  Results
+// equal div12(s0), div12(s1).
+__host__ __device__ __forceinline__ float2 div12x2(float s0, float s1) {
+  const float R = 0x1.555556p-4f;
+  const float a0 = __builtin_fabsf(s0), a1 = __builtin_fabsf(s1);
+  const bool ok0 = (a0 >= 0x1p-119f && a0 <= 0x1.fffffep+127f) || a0 == 0.0f;
+  const bool ok1 = (a1 >= 0x1p-119f && a1 <= 0x1.fffffep+127f) || a1 == 0.0f;
+  const float q0 = s0 * R, q1 = s1 * R;
+  const float r0 = __builtin_fmaf(-q0, 12.0f, s0), r1 = __builtin_fmaf(-q1, 12.0f, s1);
+  float2 q = make_float2(__builtin_copysignf(__builtin_fmaf(r0, R, q0), s0),
+                         __builtin_copysignf(__builtin_fmaf(r1, R, q1), s1));
+  if (!(ok0 && ok1)) {
+    if (!ok0) q.x = s0 / 12.0f;
+    if (!ok1) q.y = s1 / 12.0f;
+  }
+  return q;
+}
+
 // Damped leapfrog update (gpu.py:113): ((2 p1 - B p0) + c2 L) A.
 // 2*p1 is exact, so 2 p1 - (B p0) equals fma(2, p1, -(B p0)).
 template <typename T>

@@ -381,8 +381,9 @@ __global__ __launch_bounds__(kFwdThreads) void fwd_pair_kernel(Geom g, int t0, i
         const float2 um = P2[q - W2], u2m = P2[q - W], up = P2[q + W2], u2p = P2[q + W];
         const float s0 = lap_sum(u2m.x, um.x, r1.x, up.x, u2p.x, r0.x, r0.y, r1.y, r2.x);
         const float s1 = lap_sum(u2m.y, um.y, r1.y, up.y, u2p.y, r0.y, r1.x, r2.x, r2.y);
-        const float L0 = (d0 & kPInterior) ? div12(s0) : 0.0f;
-        const float L1 = (d1 & kPInterior) ? div12(s1) : 0.0f;
+        const float2 Lq = div12x2(s0, s1);
+        const float L0 = (d0 & kPInterior) ? Lq.x : 0.0f;
+        const float L1 = (d1 & kPInterior) ? Lq.y : 0.0f;
         const float2 ab0 = sponge_at(tab, d0), ab1 = sponge_at(tab, d1);
         pn[j].x = leapfrog(r1.x, pn[j].x, ab0.y, cc[j].x, L0, ab0.x);
         pn[j].y = leapfrog(r1.y, pn[j].y, ab1.y, cc[j].y, L1, ab1.x);
@@ -722,9 +723,8 @@ __global__ __launch_bounds__(kAdjThreads) void adj_pair_kernel(Geom g, int t0, i
       if (t + j * kAdjThreads < NP) {
         const uint32_t d0 = pk[j] & 0xffffu, d1 = pk[j] >> 16;
         const float2 cv = buf_load2(crs, off * 8, j * kAdjThreads * 8);
-        const float q0 = div12((cv.x * sponge_at(tab, d0).x) * m1[j].x);
-        const float q1 = div12((cv.y * sponge_at(tab, d1).x) * m1[j].y);
-        Q2[off + j * kAdjThreads] = make_float2((d0 & kPInterior) ? q0 : 0.0f, (d1 & kPInterior) ? q1 : 0.0f);
+        const float2 qq = div12x2((cv.x * sponge_at(tab, d0).x) * m1[j].x, (cv.y * sponge_at(tab, d1).x) * m1[j].y);
+        Q2[off + j * kAdjThreads] = make_float2((d0 & kPInterior) ? qq.x : 0.0f, (d1 & kPInterior) ? qq.y : 0.0f);
       }
     }
     __syncthreads();

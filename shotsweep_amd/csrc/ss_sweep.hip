@@ -457,8 +457,9 @@ __global__ __launch_bounds__(kThreads) void sweep_fwd_pair_kernel(SArgs a) {
         const float s0 = lap_sum(u2m.x, um.x, r1.x, up.x, u2p.x, r0.x, r0.y, r1.y, r2.x);
         const float s1 = lap_sum(u2m.y, um.y, r1.y, up.y, u2p.y, r0.y, r1.x, r2.x, r2.y);
         const bool in = (d & kQInt) != 0;
-        const float L0 = in ? div12(s0) : 0.0f;
-        const float L1 = in ? div12(s1) : 0.0f;
+        const float2 Lq = div12x2(s0, s1);
+        const float L0 = in ? Lq.x : 0.0f;
+        const float L1 = in ? Lq.y : 0.0f;
         const float2 ab0 = tab_at(tab, db[j] & kTabOff), ab1 = tab_at(tab, (db[j] >> 16) & kTabOff);
         float q0 = leapfrog(r1.x, pn[j].x, ab0.y, cc[j].x, L0, ab0.x);
         float q1 = leapfrog(r1.y, pn[j].y, ab1.y, cc[j].y, L1, ab1.x);
@@ -603,7 +604,8 @@ __global__ __launch_bounds__(kThreads) void sweep_adj_pair_kernel(SArgs a) {
         m2[j] = m1[j];
         m1[j] = make_float2(mu0, mu1);
         const bool in = (d & kQInt) != 0;
-        N2[p] = make_float2(in ? div12(ca[j].x * mu0) : 0.0f, in ? div12(ca[j].y * mu1) : 0.0f);
+        const float2 qq = div12x2(ca[j].x * mu0, ca[j].y * mu1);
+        N2[p] = make_float2(in ? qq.x : 0.0f, in ? qq.y : 0.0f);
       }
     }
     __syncthreads();
