@@ -2,6 +2,7 @@
 
     python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--models 64] [--seed 0] [--no-cpu]
     python bench.py --workload mixed [--models-per-size 32]    (uneven batch, see run_mixed)
+    python bench.py --workload autograd                        (differentiation, see run_autograd)
 
 Workload (one "step"): 64 seeded 70 x 70 models of mixed structure (curved
 layers, a planar fault, smoothed texture) x 3 backgrounds (surface velocity +
@@ -182,6 +183,62 @@ def run_mixed(args):
     print(json.dumps(result), flush=True)
 
 
+def run_autograd(args):
+    """Differentiation workload: misfit 0.5 * sum((F(v) - obs)^2) and its gradient through
+    the autograd Function for 16 models x 5 shots (70 x 70, nt 1000), start = smoothed truth
+    (sigma 8), checkpoints every 50 steps with recompute (the reference's scheme)."""
+    from shotsweep_amd import autograd, gpu, synth
+    sim = gpu.Sim(device='cuda', checkpoint_interval=50, history='checkpoint')
+    true = synth.mixed(16, seed=args.seed)
+    obs = sim.forward(true)[0].contiguous()
+    start = synth.smooth(true, 8.0)
+    ns = sim.geom.ns
+    for _ in range(args.warmup):
+        autograd.misfit_and_gradient(start, obs, sim)
+    torch.cuda.synchronize()
+    sim.events = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        loss, grad = autograd.misfit_and_gradient(start, obs, sim)
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) / args.steps
+    events, sim.events = sim.events, None
+    per = {}
+    for name, a, b in events:
+        per[name] = per.get(name, 0.0) + a.elapsed_time(b) / args.steps
+    dominant = max(per, key=per.get)
+    units = len(true) * ns
+    alg = BYTES_PER_CELL_UPDATE * sim.H * sim.W * sim.geom.nt * units
+    result = {
+        'metric': METRIC, 'value': units / wall, 'unit': 'shot-gradients/s', 'n_gpus': 1, 'steps': args.steps,
+        'warmup': args.warmup, 'ms_per_step': wall * 1e3, 'higher_is_better': True, 'scaling': 'weak',
+        'vs_baseline': None, 'dtype': 'f32', 'data': 'synthetic (seeded mixed-structure models)',
+        'config': {'workload': 'differentiation: misfit gradient of 16 mixed 70x70 models x 5 shots, nt 1000, '
+                               'autograd.Function, checkpoints every 50 steps with recompute',
+                   'models': len(true), 'seed': args.seed, 'misfit': loss},
+        'roofline': {'bound': 'hbm', 'kernel': dominant, 'achieved': alg / (per[dominant] * 1e-3) / 1e9,
+                     'peak': PEAK_HBM_GBS, 'unit': 'GB/s', 'frac': alg / (per[dominant] * 1e-3) / 1e9 / PEAK_HBM_GBS,
+                     'traffic': None, 'kernel_ms_per_step': per, 'algorithmic_bytes_per_direction': alg,
+                     'note': 'forward time includes the per-segment recompute (two forward passes)'},
+    }
+    if not args.no_cpu:
+        from oracle import fd
+        cores = len(os.sched_getaffinity(0))
+        torch.set_num_threads(cores)
+        osim = fd.OracleSim(checkpoint_interval=50)
+        t0 = time.perf_counter()
+        og, _ = osim.gradient(start[:1], obs[:1].cpu().numpy())
+        dt = time.perf_counter() - t0
+        result['cpu_baseline'] = {'value': ns / dt, 'unit': 'shot-gradients/s', 'cores': cores, 'kind': 'port',
+                                  'sample': f'one model (5 shot gradients), oracle FP32 eager PyTorch with '
+                                            f'checkpoints every 50 steps on {cores} threads, {dt:.1f} s'}
+        result['speedup_vs_cpu'] = result['value'] / result['cpu_baseline']['value']
+        g0 = grad[0].cpu().numpy().astype(np.float64)
+        want = og.sum(1)[0].astype(np.float64)
+        result['gradient_rel_l2_vs_cpu_oracle'] = float(np.linalg.norm(g0 - want) / np.linalg.norm(want))
+    print(json.dumps(result), flush=True)
+
+
 def copy_probe(nbytes=1 << 30, reps=3):
     """Plain device copies of known size (read nbytes, write nbytes each): the
     reference point that scales the FETCH_SIZE / WRITE_SIZE counters to bytes."""
@@ -205,8 +262,9 @@ def main():
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--cpu-models', type=int, default=8,
                     help='models in the bounded host-CPU baseline sample (x3 backgrounds x5 shots)')
-    ap.add_argument('--workload', default='batched', choices=('batched', 'mixed'),
-                    help='batched: 64 models of 70x70 (the headline metric); mixed: the uneven batch')
+    ap.add_argument('--workload', default='batched', choices=('batched', 'mixed', 'autograd'),
+                    help='batched: 64 models of 70x70 (the headline metric); mixed: the uneven batch; '
+                         'autograd: the differentiation workload')
     ap.add_argument('--models-per-size', type=int, default=32)
     ap.add_argument('--copy-probe', action='store_true', help='only run 1 GiB device copies (counter scaling)')
     args = ap.parse_args()
@@ -218,6 +276,8 @@ def main():
         sys.exit(2)
     if args.workload == 'mixed':
         return run_mixed(args)
+    if args.workload == 'autograd':
+        return run_autograd(args)
     from shotsweep_amd import gpu
 
     dev = torch.device('cuda')

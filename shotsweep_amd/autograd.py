@@ -61,4 +61,4 @@ def misfit_and_gradient(v, obs, sim):
     obs = torch.as_tensor(obs, dtype=sim.dtype, device=sim.device)
     loss = misfit(v, obs, sim)
     loss.backward()
-    return float(loss), v.grad.detach()
+    return float(loss.detach()), v.grad.detach()
