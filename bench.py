@@ -260,7 +260,7 @@ def main():
     ap.add_argument('--seed', type=int, default=0)
     ap.add_argument('--history', default='auto', choices=('auto', 'full', 'checkpoint'))
     ap.add_argument('--no-cpu', action='store_true')
-    ap.add_argument('--cpu-models', type=int, default=8,
+    ap.add_argument('--cpu-models', type=int, default=20,
                     help='models in the bounded host-CPU baseline sample (x3 backgrounds x5 shots)')
     ap.add_argument('--workload', default='batched', choices=('batched', 'mixed', 'autograd'),
                     help='batched: 64 models of 70x70 (the headline metric); mixed: the uneven batch; '
