@@ -491,10 +491,13 @@ __global__ __launch_bounds__(kAdjThreads) void adj_resident_kernel(Geom g, int t
   __syncthreads();
   constexpr int D = K < kPrefetch ? K : kPrefetch;
   static_assert(K % D == 0, "slot count must be a multiple of the prefetch distance");
-  float Lb[D];
+  float Lb[D], Cb[D];  // history and c2 of the cells D ahead
 #pragma unroll
-  for (int d = 0; d < D; ++d)
-    Lb[d] = (t + d * kAdjThreads < HW) ? buf_load(hrs, t * 4, ((t1 - 1 - t0) * HW + d * kAdjThreads) * 4) : 0.0f;
+  for (int d = 0; d < D; ++d) {
+    const bool live = t + d * kAdjThreads < HW;
+    Lb[d] = live ? buf_load(hrs, t * 4, ((t1 - 1 - t0) * HW + d * kAdjThreads) * 4) : 0.0f;
+    Cb[d] = live ? buf_load(crs, t * 4, d * kAdjThreads * 4) : 0.0f;
+  }
   for (int it = t1 - 1; it >= t0; --it) {
     const int slot = it - t0;
     const float wt = w[it];
@@ -544,9 +547,15 @@ __global__ __launch_bounds__(kAdjThreads) void adj_resident_kernel(Geom g, int t
 #pragma unroll
     for (int j = 0; j < K; ++j) {
       group_fence(j, off, pk);
+      const float cv = Cb[j % D];
+      {
+        // c2 is the same every step: after the last cell, load the first cells of the next step
+        const int jn = j + D < K ? j + D : j + D - K;
+        if (t + jn * kAdjThreads < HW) Cb[j % D] = buf_load(crs, off * 4, jn * kAdjThreads * 4);
+      }
       if (t + j * kAdjThreads < HW) {
         const uint32_t d = desc_of(pk, j);
-        const float q = div12((buf_load(crs, off * 4, j * kAdjThreads * 4) * sponge(tab, d).x) * m1[j]);
+        const float q = div12((cv * sponge(tab, d).x) * m1[j]);
         Q[off + j * kAdjThreads] = (d & kInterior) ? q : 0.0f;
       }
     }
@@ -652,11 +661,13 @@ __global__ __launch_bounds__(kAdjThreads) void adj_pair_kernel(Geom g, int t0, i
   __syncthreads();
   constexpr int D = pair_prefetch<KP>();
   static_assert(KP % D == 0, "pair count must be a multiple of the prefetch distance");
-  float2 Lb[D];
+  float2 Lb[D], Cb[D];  // history and c2 of the pairs D ahead
 #pragma unroll
-  for (int d = 0; d < D; ++d)
-    Lb[d] = (t + d * kAdjThreads < NP) ? buf_load2(hrs, t * 8, ((t1 - 1 - t0) * HW + 2 * d * kAdjThreads) * 4)
-                                       : make_float2(0.0f, 0.0f);
+  for (int d = 0; d < D; ++d) {
+    const bool live = t + d * kAdjThreads < NP;
+    Lb[d] = live ? buf_load2(hrs, t * 8, ((t1 - 1 - t0) * HW + 2 * d * kAdjThreads) * 4) : make_float2(0.0f, 0.0f);
+    Cb[d] = live ? buf_load2(crs, t * 8, d * kAdjThreads * 8) : make_float2(0.0f, 0.0f);
+  }
   for (int it = t1 - 1; it >= t0; --it) {
     const int slot = it - t0;
     const float wt = w[it];
@@ -720,9 +731,14 @@ __global__ __launch_bounds__(kAdjThreads) void adj_pair_kernel(Geom g, int t0, i
 #pragma unroll
     for (int j = 0; j < KP; ++j) {
       asm volatile("" : "+v"(off), "+v"(pk[j]));
+      const float2 cv = Cb[j % D];
+      {
+        // c2 is the same every step: after the last pair, load the first pairs of the next step
+        const int jn = j + D < KP ? j + D : j + D - KP;
+        if (t + jn * kAdjThreads < NP) Cb[j % D] = buf_load2(crs, off * 8, jn * kAdjThreads * 8);
+      }
       if (t + j * kAdjThreads < NP) {
         const uint32_t d0 = pk[j] & 0xffffu, d1 = pk[j] >> 16;
-        const float2 cv = buf_load2(crs, off * 8, j * kAdjThreads * 8);
         const float2 qq = div12x2((cv.x * sponge_at(tab, d0).x) * m1[j].x, (cv.y * sponge_at(tab, d1).x) * m1[j].y);
         Q2[off + j * kAdjThreads] = make_float2((d0 & kPInterior) ? qq.x : 0.0f, (d1 & kPInterior) ? qq.y : 0.0f);
       }

@@ -563,6 +563,13 @@ __global__ __launch_bounds__(kThreads) void sweep_adj_pair_kernel(SArgs a) {
   const __amdgpu_buffer_rsrc_t rrs = make_rsrc(G.res + (int64_t)v.prop * G.nt * G.nx, (int64_t)G.nt * G.nx);
   const int rec0 = r * W + pad;
   const float sdv = G.amp[v.prop];
+  // Laplacian history of the pairs D ahead.  The loads are unconditional: off
+  // the own tile and past the last pair (descriptor 0) they are masked and read 0.
+  constexpr int D = KP % 2 == 0 ? 2 : 1;
+  float2 Lb[D];
+#pragma unroll
+  for (int j = 0; j < D; ++j)
+    Lb[j] = buf_load2(hrs, (da[j] & kQOwn) ? (int)(da[j] & kOffMask) * 4 : kNoAccess, (hi - 1) * HW * 4);
   __syncthreads();
 
   for (int st = 0; st < n; ++st) {
@@ -573,10 +580,16 @@ __global__ __launch_bounds__(kThreads) void sweep_adj_pair_kernel(SArgs a) {
 #pragma unroll
     for (int j = 0; j < KP; ++j) {
       const int p = t + j * kThreads;
+      const float2 L = Lb[j % D];
+      {
+        // wraps to the first pairs of the next step (the last step reloads its own: unused)
+        const int jn = j + D < KP ? j + D : j + D - KP;
+        const int itn = (j + D < KP || st + 1 == n) ? it : it - 1;
+        Lb[j % D] = buf_load2(hrs, (da[jn] & kQOwn) ? (int)(da[jn] & kOffMask) * 4 : kNoAccess, itn * HW * 4);
+      }
       if (p < NPE) {
         const uint32_t d = da[j];
         const int gi = (int)(d & kOffMask);
-        const float2 L = buf_load2(hrs, (d & kQOwn) ? gi * 4 : kNoAccess, it * HW * 4);  // 0 off the own tile
         const float2 r0 = C2[p - 1], r1 = C2[p], r2 = C2[p + 1];
         const float2 um = C2[p - EW2], u2m = C2[p - EW], up = C2[p + EW2], u2p = C2[p + EW];
         const float lt0 = lapt_sum(r1.x, u2p.x, up.x, um.x, u2m.x, r2.x, r1.y, r0.y, r0.x);
