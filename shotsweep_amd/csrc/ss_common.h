@@ -76,33 +76,33 @@ __host__ __device__ __forceinline__ T lapt_sum(T qc, T qzp2, T qzp1, T qzm1, T q
   return s;
 }
 
-// Correctly rounded s/12: the reciprocal product plus one FMA correction
-// equals IEEE division for every finite s with |s| >= 2^-119, and for s = +-0
-// once the sign is restored (checked on the host over every mantissa and the
-// whole small-magnitude range by tests/native/div12_check.cpp).  Tiny nonzero
-// or non-finite s take IEEE division; exact zeros, common ahead of the
-// wavefront, stay on the fast path.
-__host__ __device__ __forceinline__ float div12(float s) {
+// Correctly rounded s/12 from the reciprocal product and one FMA correction.
+// Whenever that result is zero or normal it equals IEEE division; it can be
+// wrong only when it is subnormal (|s| below about 2^-122), and it is NaN for
+// infinite s.  Those cases take IEEE division, chosen by one class test on the
+// result (checked on the host for all 2^32 inputs by tests/native/div12_check.cpp).
+// Exact zeros, common ahead of the wavefront, stay on the fast path.
+constexpr int kZeroOrNormal = 0x168;  // __builtin_isfpclass: -normal, -zero, +zero, +normal
+
+__host__ __device__ __forceinline__ float div12_fast(float s) {
   const float R = 0x1.555556p-4f;  // RN(1/12)
-  const float a = __builtin_fabsf(s);
-  if (!((a >= 0x1p-119f && a <= 0x1.fffffep+127f) || a == 0.0f)) return s / 12.0f;
   const float q0 = s * R;
   const float r = __builtin_fmaf(-q0, 12.0f, s);
   return __builtin_copysignf(__builtin_fmaf(r, R, q0), s);
 }
 
-// div12 of two values with one branch: both take the fast path unless either
+__host__ __device__ __forceinline__ float div12(float s) {
+  float q = div12_fast(s);
+  if (!__builtin_isfpclass(q, kZeroOrNormal)) q = s / 12.0f;
+  return q;
+}
+
+// div12 of two values with one branch: both keep the fast result unless either
 // needs IEEE division, and then each falls back only if it needs to.  Results
 // equal div12(s0), div12(s1).
 __host__ __device__ __forceinline__ float2 div12x2(float s0, float s1) {
-  const float R = 0x1.555556p-4f;
-  const float a0 = __builtin_fabsf(s0), a1 = __builtin_fabsf(s1);
-  const bool ok0 = (a0 >= 0x1p-119f && a0 <= 0x1.fffffep+127f) || a0 == 0.0f;
-  const bool ok1 = (a1 >= 0x1p-119f && a1 <= 0x1.fffffep+127f) || a1 == 0.0f;
-  const float q0 = s0 * R, q1 = s1 * R;
-  const float r0 = __builtin_fmaf(-q0, 12.0f, s0), r1 = __builtin_fmaf(-q1, 12.0f, s1);
-  float2 q = make_float2(__builtin_copysignf(__builtin_fmaf(r0, R, q0), s0),
-                         __builtin_copysignf(__builtin_fmaf(r1, R, q1), s1));
+  float2 q = make_float2(div12_fast(s0), div12_fast(s1));
+  const bool ok0 = __builtin_isfpclass(q.x, kZeroOrNormal), ok1 = __builtin_isfpclass(q.y, kZeroOrNormal);
   if (!(ok0 && ok1)) {
     if (!ok0) q.x = s0 / 12.0f;
     if (!ok1) q.y = s1 / 12.0f;

@@ -90,19 +90,47 @@ __device__ __forceinline__ uint32_t cell_desc(int i, int H, int W, int nz, int n
 }
 
 // Cell descriptor of the paired kernels (16 bits): bit 0 Laplacian interior,
-// bit 1 model cell, bits 3-13 the byte offset of the cell's sponge-table entry
-// (index * 8).  Model cells sit at sponge depth 0, i.e. entry 0 (A = B = 1), so
-// a table lookup is one mask and one add, with no select.
-constexpr uint32_t kPInterior = 1u, kPModel = 2u, kPTabMask = 0x3ff8u;
+// bit 1 model cell, and the byte offset of the cell's sponge-table entry: bits
+// 3-13 for the forward's 8-byte (A, B) entries, bits 4-14 for the adjoint's
+// 16-byte (A, 2A, A B, 0) entries.  Model cells sit at sponge depth 0, i.e.
+// entry 0 (A = B = 1).  The table starts at LDS address 0, so a lookup is one
+// mask and the load itself.
+constexpr uint32_t kPInterior = 1u, kPModel = 2u, kPTabMask = 0x3ff8u, kPTab4Mask = 0x7ff0u;
+template <int SHIFT>
 __device__ __forceinline__ uint32_t pair_cell_desc(int i, int H, int W, int nz, int nx, int pad) {
   const int z = i / W, x = i - z * W;
-  uint32_t v = (uint32_t)(sponge_dist(z, H, pad) * (pad + 1) + sponge_dist(x, W, pad)) << 3;
+  uint32_t v = (uint32_t)(sponge_dist(z, H, pad) * (pad + 1) + sponge_dist(x, W, pad)) << SHIFT;
   if (z >= pad && z < pad + nz && x >= pad && x < pad + nx) v |= kPModel;
   if (z >= 2 && z < H - 2 && x >= 2 && x < W - 2) v |= kPInterior;
   return v;
 }
-__device__ __forceinline__ float2 sponge_at(const float2 *tab, uint32_t d) {
-  return *reinterpret_cast<const float2 *>(reinterpret_cast<const char *>(tab) + (d & kPTabMask));
+// The paired kernels have no static LDS (checked on the host before launch), so
+// their dynamic LDS, and the sponge table at its start, begin at LDS address 0:
+// a lookup reads the LDS address given by the descriptor's masked byte offset.
+typedef float ss_v2f __attribute__((ext_vector_type(2)));
+typedef __attribute__((address_space(3))) const float lds_cfloat;
+typedef __attribute__((address_space(3))) const ss_v2f lds_cv2f;
+__device__ __forceinline__ float2 sponge_at(uint32_t d) {
+  const ss_v2f v = *reinterpret_cast<lds_cv2f *>((uintptr_t)(d & kPTabMask));
+  return make_float2(v.x, v.y);
+}
+// (A, 2A, A B) of an adjoint table entry (its fourth word is padding)
+__device__ __forceinline__ float3 sponge4_at(uint32_t d) {
+  const uintptr_t a = d & kPTab4Mask;
+  const ss_v2f v = *reinterpret_cast<lds_cv2f *>(a);
+  return make_float3(v.x, v.y, *reinterpret_cast<lds_cfloat *>(a + 8));
+}
+__device__ __forceinline__ float sponge4_a(uint32_t d) {
+  return *reinterpret_cast<lds_cfloat *>((uintptr_t)(d & kPTab4Mask));
+}
+
+// v if descriptor bit `bit` is set, else +0 (sign-extended bit as an AND mask).
+// The empty asm keeps the compiler from turning the AND back into a compare and select.
+template <int BIT>
+__device__ __forceinline__ float keep_if(float v, uint32_t d) {
+  uint32_t m = (uint32_t)((int32_t)(d << (31 - BIT)) >> 31);
+  asm("" : "+v"(m));
+  return __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, v) & m);
 }
 
 // Per-cell descriptors are loop-invariant; left visible, the compiler hoists
@@ -135,6 +163,17 @@ __device__ void fill_table(float2 *tab, const float *A, const float *B, int W, i
   }
 }
 
+// Table (dz, dx) -> (A, 2A, A*B, 0) of the paired adjoint (2A is exact).
+__device__ void fill_table4(float4 *tab, const float *A, const float *B, int W, int pad) {
+  const int n = (pad + 1) * (pad + 1);
+  for (int k = threadIdx.x; k < n; k += blockDim.x) {
+    const int dz = k / (pad + 1), dx = k - dz * (pad + 1);
+    const int c = (pad - dz) * W + (pad - dx);
+    const float a = A[c], b = B[c];
+    tab[k] = make_float4(a, 2.0f * a, a * b, 0.0f);
+  }
+}
+
 // Workgroup -> propagation.  Workgroups are dealt round-robin to the 8 XCDs
 // (blockIdx % 8), so XCD x receives blocks x, x+8, ...; mapping those to the
 // contiguous propagation range [x*per, (x+1)*per) keeps the shots of one
@@ -161,6 +200,16 @@ size_t fwd_lds_bytes(const Geom &g) {
 
 size_t adj_lds_bytes(const Geom &g) {
   return fwd_lds_bytes(g) + (size_t)g.nz * g.nx * sizeof(float);
+}
+
+// Paired kernels (even plane width): [sponge table | guard | plane | guard | illumination (adjoint)].
+constexpr int pair_guard(int W) { return 2 * W + 2; }
+size_t fwd_pair_lds_bytes(const Geom &g) {
+  return (size_t)(g.pad + 1) * (g.pad + 1) * sizeof(float2) + (size_t)(g.H * g.W + 2 * pair_guard(g.W)) * sizeof(float);
+}
+size_t adj_pair_lds_bytes(const Geom &g) {
+  return (size_t)(g.pad + 1) * (g.pad + 1) * sizeof(float4) +
+         (size_t)(g.H * g.W + 2 * pair_guard(g.W) + g.nz * g.nx) * sizeof(float);
 }
 
 // NZ, NX, PAD > 0 and ROW >= 0 fix the geometry at compile time (0 / -1: runtime).
@@ -309,17 +358,17 @@ __global__ __launch_bounds__(kFwdThreads) void fwd_pair_kernel(Geom g, int t0, i
   constexpr int HW = H * W, NP = HW / 2, W2 = W / 2;
   static_assert(W % 2 == 0 && HW % 2 == 0, "pairs need an even plane width");
   static_assert((pad + 1) * (pad + 1) * 8 <= (int)kPTabMask + 8, "sponge-table offsets must fit the descriptor");
-  constexpr int guard = 2 * W + 2;
-  float *P = lds + guard;
+  constexpr int guard = pair_guard(W), tabf = (pad + 1) * (pad + 1) * 2;
+  float2 *tab = reinterpret_cast<float2 *>(lds);  // at LDS address 0: lookups need no base add
+  float *P = lds + tabf + guard;
   float2 *P2 = reinterpret_cast<float2 *>(P);
-  float2 *tab = reinterpret_cast<float2 *>(lds + table_offset(H, W));
   const int64_t prop = xcd_prop(g);
   if (prop < 0) return;  // padding workgroup (whole workgroup exits before any barrier)
   const int b = (int)(prop / g.ns), s = (int)(prop - (int64_t)b * g.ns);
   const int t = threadIdx.x;
 
   for (int k = t; k < guard; k += kFwdThreads) {
-    lds[k] = 0.0f;
+    P[k - guard] = 0.0f;
     P[HW + k] = 0.0f;
   }
   fill_table(tab, A, B, W, pad, 0);
@@ -338,7 +387,7 @@ __global__ __launch_bounds__(kFwdThreads) void fwd_pair_kernel(Geom g, int t0, i
       pn[j] = p0p[q];
       P2[q] = p1p[q];
       cc[j] = c2b[q];
-      pk[j] = pair_cell_desc(2 * q, H, W, nz, nx, pad) | (pair_cell_desc(2 * q + 1, H, W, nz, nx, pad) << 16);
+      pk[j] = pair_cell_desc<3>(2 * q, H, W, nz, nx, pad) | (pair_cell_desc<3>(2 * q + 1, H, W, nz, nx, pad) << 16);
     }
   }
   constexpr int rec0 = (pad + row) * W + pad;
@@ -382,9 +431,8 @@ __global__ __launch_bounds__(kFwdThreads) void fwd_pair_kernel(Geom g, int t0, i
         const float s0 = lap_sum(u2m.x, um.x, r1.x, up.x, u2p.x, r0.x, r0.y, r1.y, r2.x);
         const float s1 = lap_sum(u2m.y, um.y, r1.y, up.y, u2p.y, r0.y, r1.x, r2.x, r2.y);
         const float2 Lq = div12x2(s0, s1);
-        const float L0 = (d0 & kPInterior) ? Lq.x : 0.0f;
-        const float L1 = (d1 & kPInterior) ? Lq.y : 0.0f;
-        const float2 ab0 = sponge_at(tab, d0), ab1 = sponge_at(tab, d1);
+        const float L0 = keep_if<0>(Lq.x, pk[j]), L1 = keep_if<16>(Lq.y, pk[j]);  // interior flags
+        const float2 ab0 = sponge_at(d0), ab1 = sponge_at(d1);
         pn[j].x = leapfrog(r1.x, pn[j].x, ab0.y, cc[j].x, L0, ab0.x);
         pn[j].y = leapfrog(r1.y, pn[j].y, ab1.y, cc[j].y, L1, ab1.x);
         if (hist) buf_store2(make_float2(L0, L1), hrs, off * 8, (slot * HW + 2 * j * kFwdThreads) * 4);
@@ -599,12 +647,12 @@ __global__ __launch_bounds__(kAdjThreads) void adj_pair_kernel(Geom g, int t0, i
   constexpr int H = nz + 2 * pad, W = nx + 2 * pad;
   constexpr int HW = H * W, NP = HW / 2, W2 = W / 2;
   static_assert(W % 2 == 0 && nx % 2 == 0 && pad % 2 == 0, "pairs need even widths");
-  static_assert((pad + 1) * (pad + 1) * 8 <= (int)kPTabMask + 8, "sponge-table offsets must fit the descriptor");
-  constexpr int guard = 2 * W + 2;
-  float *Q = lds + guard;
+  static_assert((pad + 1) * (pad + 1) * 16 <= (int)kPTab4Mask + 16, "sponge-table offsets must fit the descriptor");
+  constexpr int guard = pair_guard(W), tabf = (pad + 1) * (pad + 1) * 4;
+  float4 *tab = reinterpret_cast<float4 *>(lds);  // at LDS address 0: lookups need no base add
+  float *Q = lds + tabf + guard;
   float2 *Q2 = reinterpret_cast<float2 *>(Q);
-  float2 *tab = reinterpret_cast<float2 *>(lds + table_offset(H, W));
-  float *il = reinterpret_cast<float *>(tab + (pad + 1) * (pad + 1));
+  float *il = Q + HW + guard;
   float2 *il2 = reinterpret_cast<float2 *>(il);
   const int64_t prop = xcd_prop(g);
   if (prop < 0) return;
@@ -612,17 +660,17 @@ __global__ __launch_bounds__(kAdjThreads) void adj_pair_kernel(Geom g, int t0, i
   const int t = threadIdx.x;
   constexpr int nzx = nz * nx;
   for (int k = t; k < guard; k += kAdjThreads) {
-    lds[k] = 0.0f;
+    Q[k - guard] = 0.0f;
     Q[HW + k] = 0.0f;
   }
-  fill_table(tab, A, B, W, pad, 1);
+  fill_table4(tab, A, B, W, pad);
   for (int k = t; k < nzx; k += kAdjThreads) il[k] = illumg[prop * nzx + k];
   __syncthreads();
   uint32_t pk[KP];
 #pragma unroll
   for (int j = 0; j < KP; ++j) {
     const int q = t + j * kAdjThreads;
-    pk[j] = q < NP ? (pair_cell_desc(2 * q, H, W, nz, nx, pad) | (pair_cell_desc(2 * q + 1, H, W, nz, nx, pad) << 16))
+    pk[j] = q < NP ? (pair_cell_desc<4>(2 * q, H, W, nz, nx, pad) | (pair_cell_desc<4>(2 * q + 1, H, W, nz, nx, pad) << 16))
                    : 0u;
   }
   const __amdgpu_buffer_rsrc_t srs1 = make_rsrc(m1g + prop * HW, HW);
@@ -645,8 +693,8 @@ __global__ __launch_bounds__(kAdjThreads) void adj_pair_kernel(Geom g, int t0, i
     if (q < NP) {
       const uint32_t d0 = pk[j] & 0xffffu, d1 = pk[j] >> 16;
       const float2 cv = buf_load2(crs, t * 8, j * kAdjThreads * 8);
-      Q2[q] = make_float2((d0 & kPInterior) ? div12((cv.x * sponge_at(tab, d0).x) * m1[j].x) : 0.0f,
-                          (d1 & kPInterior) ? div12((cv.y * sponge_at(tab, d1).x) * m1[j].y) : 0.0f);
+      Q2[q] = make_float2((d0 & kPInterior) ? div12((cv.x * sponge4_a(d0)) * m1[j].x) : 0.0f,
+                          (d1 & kPInterior) ? div12((cv.y * sponge4_a(d1)) * m1[j].y) : 0.0f);
     }
   }
   constexpr int rec0 = (pad + row) * W + pad, rec1 = rec0 + nx;
@@ -693,13 +741,13 @@ __global__ __launch_bounds__(kAdjThreads) void adj_pair_kernel(Geom g, int t0, i
         const float2 um = Q2[q - W2], u2m = Q2[q - W], up = Q2[q + W2], u2p = Q2[q + W];
         const float lt0 = lapt_sum(r1.x, u2p.x, up.x, um.x, u2m.x, r2.x, r1.y, r0.y, r0.x);
         const float lt1 = lapt_sum(r1.y, u2p.y, up.y, um.y, u2m.y, r2.y, r2.x, r1.x, r0.y);
-        const float2 aab0 = sponge_at(tab, d0), aab1 = sponge_at(tab, d1);
-        float mu0 = (2.0f * aab0.x) * m1[j].x;
-        float mu1 = (2.0f * aab1.x) * m1[j].y;
+        const float3 aab0 = sponge4_at(d0), aab1 = sponge4_at(d1);  // (A, 2A, A B)
+        float mu0 = aab0.y * m1[j].x;
+        float mu1 = aab1.y * m1[j].y;
         mu0 = mu0 + lt0;
         mu1 = mu1 + lt1;
-        mu0 = mu0 - aab0.y * m2[j].x;
-        mu1 = mu1 - aab1.y * m2[j].y;
+        mu0 = mu0 - aab0.z * m2[j].x;
+        mu1 = mu1 - aab1.z * m2[j].y;
         if (j * kAdjThreads < rec1 / 2 && (j + 1) * kAdjThreads > rec0 / 2) {
           if (q >= rec0 / 2 && q < rec1 / 2) {
             const float2 rv = buf_load2(rrs, (2 * q - rec0) * 4, it * nx * 4);
@@ -739,8 +787,8 @@ __global__ __launch_bounds__(kAdjThreads) void adj_pair_kernel(Geom g, int t0, i
       }
       if (t + j * kAdjThreads < NP) {
         const uint32_t d0 = pk[j] & 0xffffu, d1 = pk[j] >> 16;
-        const float2 qq = div12x2((cv.x * sponge_at(tab, d0).x) * m1[j].x, (cv.y * sponge_at(tab, d1).x) * m1[j].y);
-        Q2[off + j * kAdjThreads] = make_float2((d0 & kPInterior) ? qq.x : 0.0f, (d1 & kPInterior) ? qq.y : 0.0f);
+        const float2 qq = div12x2((cv.x * sponge4_a(d0)) * m1[j].x, (cv.y * sponge4_a(d1)) * m1[j].y);
+        Q2[off + j * kAdjThreads] = make_float2(keep_if<0>(qq.x, pk[j]), keep_if<16>(qq.y, pk[j]));
       }
     }
     __syncthreads();
@@ -795,6 +843,13 @@ int dispatch_adj(const Geom &g, Fn &&fn) {
   return fn(integral_constant<int, kAdjMaxSlots>(), std::false_type());
 }
 
+// The paired kernels address their sponge table at LDS address 0, which holds
+// only when the kernel has no static LDS ahead of its dynamic LDS.
+bool no_static_lds(const void *kern) {
+  hipFuncAttributes attr;
+  return hipFuncGetAttributes(&attr, kern) == hipSuccess && attr.sharedSizeBytes == 0;
+}
+
 }  // namespace
 
 int64_t resident_max_cells() {
@@ -806,13 +861,14 @@ bool resident_fits(const Geom &g) {
   if (slots_for(g, kFwdThreads) > kFwdMaxSlots || slots_for(g, kAdjThreads) > kAdjMaxSlots) return false;
   if ((g.pad + 1) * (g.pad + 1) > (int)kIdxMask + 1 || g.nz * g.nx > (int)kIdxMask + 1) return false;
   if ((int64_t)g.nt * g.H * g.W > 0x1fffffff) return false;  // history slots addressed with 32-bit byte offsets
-  return adj_lds_bytes(g) <= (size_t)device_lds_limit();
+  const size_t need = is_openfwi(g) ? std::max(adj_pair_lds_bytes(g), fwd_pair_lds_bytes(g)) : adj_lds_bytes(g);
+  return need <= (size_t)device_lds_limit();
 }
 
 int forward_resident(const Geom &g, int t0, int t1, const float *c2, const float *A, const float *B, const float *w,
                      const float *amp, float *p0, float *p1, float *gathers, float *hist, float *ckpt,
                      int ckpt_every, hipStream_t st) {
-  const size_t lds = fwd_lds_bytes(g);
+  const size_t lds = is_openfwi(g) ? fwd_pair_lds_bytes(g) : fwd_lds_bytes(g);
   return dispatch_fwd(g, [&](auto kc, auto fixed) -> int {
     constexpr bool F = decltype(fixed)::value;
     // the fixed 150 x 150 plane runs the paired kernel (11 pairs per thread)
@@ -820,6 +876,7 @@ int forward_resident(const Geom &g, int t0, int t1, const float *c2, const float
                   : fwd_resident_kernel<decltype(kc)::value, F ? 70 : 0, F ? 70 : 0, F ? 40 : 0, F ? 1 : -1>;
     hipError_t e = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return hip_status(e, "forward_resident attribute");
+    if (F && !no_static_lds((const void *)kern)) return hip_status(hipErrorInvalidValue, "forward_resident static LDS");
     kern<<<dim3((unsigned)xcd_blocks((int64_t)g.nb * g.ns)), dim3(kFwdThreads), lds, st>>>(g, t0, t1, c2, A, B, w, amp, p0,
                                                                                  p1, gathers, hist, ckpt, ckpt_every);
     return hip_status(hipGetLastError(), "forward_resident launch");
@@ -829,7 +886,7 @@ int forward_resident(const Geom &g, int t0, int t1, const float *c2, const float
 int adjoint_resident(const Geom &g, int t0, int t1, const float *c2, const float *A, const float *B, const float *w,
                      const float *sd, const float *res, const float *hist, float *m1, float *m2, float *gacc,
                      float *illum, float *src_grad, hipStream_t st) {
-  const size_t lds = adj_lds_bytes(g);
+  const size_t lds = is_openfwi(g) ? adj_pair_lds_bytes(g) : adj_lds_bytes(g);
   return dispatch_adj(g, [&](auto kc, auto fixed) -> int {
     constexpr bool F = decltype(fixed)::value;
     // the fixed 150 x 150 plane runs the paired kernel (22 pairs per thread)
@@ -837,6 +894,7 @@ int adjoint_resident(const Geom &g, int t0, int t1, const float *c2, const float
                   : adj_resident_kernel<decltype(kc)::value, F ? 70 : 0, F ? 70 : 0, F ? 40 : 0, F ? 1 : -1>;
     hipError_t e = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return hip_status(e, "adjoint_resident attribute");
+    if (F && !no_static_lds((const void *)kern)) return hip_status(hipErrorInvalidValue, "adjoint_resident static LDS");
     kern<<<dim3((unsigned)xcd_blocks((int64_t)g.nb * g.ns)), dim3(kAdjThreads), lds, st>>>(g, t0, t1, c2, A, B, w, sd, res,
                                                                                  hist, m1, m2, gacc, illum, src_grad);
     return hip_status(hipGetLastError(), "adjoint_resident launch");
