@@ -48,6 +48,37 @@ __device__ __forceinline__ float2 buf_load2(__amdgpu_buffer_rsrc_t r, int voff, 
   return make_float2(__builtin_bit_cast(float, (uint32_t)u), __builtin_bit_cast(float, (uint32_t)(u >> 32)));
 }
 
+// Sponge tables at LDS address 0.  Kernels without static LDS (checked on the
+// host before launch) start their dynamic LDS at address 0, so a table placed
+// first is read at the LDS address equal to the entry's byte offset, with no
+// base add.  Forward tables hold (A, B) in 8-byte entries, adjoint tables
+// (A, 2A, A*B, 0) in 16-byte entries (2A is exact).
+typedef float ss_v2f __attribute__((ext_vector_type(2)));
+typedef __attribute__((address_space(3))) const float ss_lds_cfloat;
+typedef __attribute__((address_space(3))) const ss_v2f ss_lds_cv2f;
+__device__ __forceinline__ float2 lds_table2(uint32_t byte_off) {
+  const ss_v2f v = *reinterpret_cast<ss_lds_cv2f *>((uintptr_t)byte_off);
+  return make_float2(v.x, v.y);
+}
+// (A, 2A, A*B) of a 16-byte adjoint entry
+__device__ __forceinline__ float3 lds_table3(uint32_t byte_off) {
+  const ss_v2f v = *reinterpret_cast<ss_lds_cv2f *>((uintptr_t)byte_off);
+  return make_float3(v.x, v.y, *reinterpret_cast<ss_lds_cfloat *>((uintptr_t)byte_off + 8));
+}
+__device__ __forceinline__ float lds_table_a(uint32_t byte_off) {
+  return *reinterpret_cast<ss_lds_cfloat *>((uintptr_t)byte_off);
+}
+__device__ __forceinline__ float4 table4_entry(float a, float b) { return make_float4(a, 2.0f * a, a * b, 0.0f); }
+
+// v if bit BIT of d is set, else +0: the sign-extended bit as an AND mask.  The
+// empty asm keeps the compiler from turning the AND back into compare + select.
+template <int BIT>
+__device__ __forceinline__ float keep_if(float v, uint32_t d) {
+  uint32_t m = (uint32_t)((int32_t)(d << (31 - BIT)) >> 31);
+  asm("" : "+v"(m));
+  return __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, v) & m);
+}
+
 // floor(n / d) with magic = 0xffffffff / d + 1, exact whenever n * d < 2^32:
 // n * magic / 2^32 exceeds n / d by less than n / 2^32 < 1/d, which is no more
 // than the distance from n / d up to the next integer.

@@ -104,34 +104,9 @@ __device__ __forceinline__ uint32_t pair_cell_desc(int i, int H, int W, int nz, 
   if (z >= 2 && z < H - 2 && x >= 2 && x < W - 2) v |= kPInterior;
   return v;
 }
-// The paired kernels have no static LDS (checked on the host before launch), so
-// their dynamic LDS, and the sponge table at its start, begin at LDS address 0:
-// a lookup reads the LDS address given by the descriptor's masked byte offset.
-typedef float ss_v2f __attribute__((ext_vector_type(2)));
-typedef __attribute__((address_space(3))) const float lds_cfloat;
-typedef __attribute__((address_space(3))) const ss_v2f lds_cv2f;
-__device__ __forceinline__ float2 sponge_at(uint32_t d) {
-  const ss_v2f v = *reinterpret_cast<lds_cv2f *>((uintptr_t)(d & kPTabMask));
-  return make_float2(v.x, v.y);
-}
-// (A, 2A, A B) of an adjoint table entry (its fourth word is padding)
-__device__ __forceinline__ float3 sponge4_at(uint32_t d) {
-  const uintptr_t a = d & kPTab4Mask;
-  const ss_v2f v = *reinterpret_cast<lds_cv2f *>(a);
-  return make_float3(v.x, v.y, *reinterpret_cast<lds_cfloat *>(a + 8));
-}
-__device__ __forceinline__ float sponge4_a(uint32_t d) {
-  return *reinterpret_cast<lds_cfloat *>((uintptr_t)(d & kPTab4Mask));
-}
-
-// v if descriptor bit `bit` is set, else +0 (sign-extended bit as an AND mask).
-// The empty asm keeps the compiler from turning the AND back into a compare and select.
-template <int BIT>
-__device__ __forceinline__ float keep_if(float v, uint32_t d) {
-  uint32_t m = (uint32_t)((int32_t)(d << (31 - BIT)) >> 31);
-  asm("" : "+v"(m));
-  return __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, v) & m);
-}
+__device__ __forceinline__ float2 sponge_at(uint32_t d) { return lds_table2(d & kPTabMask); }
+__device__ __forceinline__ float3 sponge4_at(uint32_t d) { return lds_table3(d & kPTab4Mask); }
+__device__ __forceinline__ float sponge4_a(uint32_t d) { return lds_table_a(d & kPTab4Mask); }
 
 // Per-cell descriptors are loop-invariant; left visible, the compiler hoists
 // every value derived from them (table addresses, flag masks) out of the time
@@ -170,7 +145,7 @@ __device__ void fill_table4(float4 *tab, const float *A, const float *B, int W, 
     const int dz = k / (pad + 1), dx = k - dz * (pad + 1);
     const int c = (pad - dz) * W + (pad - dx);
     const float a = A[c], b = B[c];
-    tab[k] = make_float4(a, 2.0f * a, a * b, 0.0f);
+    tab[k] = table4_entry(a, b);
   }
 }
 

@@ -72,7 +72,9 @@ struct SGroup {
   int src[SS_MAX_SHOTS];
 };
 
-// LDS layout (floats): [guard | plane 0 | guard | plane 1 | guard | sponge table | illumination]
+// LDS layout (floats): [sponge table | guard | plane 0 | guard | plane 1 | guard | illumination].
+// The table starts at LDS address 0 (tab_off 0): 8-byte (A, B) or (A, A*B)
+// entries, or the paired adjoint's 16-byte (A, 2A, A*B, 0) entries.
 struct SArgs {
   SGroup g[kMaxGroups];
   int ngroups, k, nactive;
@@ -373,11 +375,16 @@ __global__ __launch_bounds__(kThreads) void sweep_adj_kernel(SArgs a) {
 // words per pair: A = plane offset of the first cell | flags (one source flag
 // per cell), B = the two cells' sponge-table byte offsets.  Per cell the
 // arithmetic is that of the cell kernels above.
-constexpr uint32_t kQInt = 1u << 17, kQOwn = 1u << 18, kQRow = 1u << 19, kQSrc0 = 1u << 20, kQSrc1 = 1u << 21;
-constexpr uint32_t kTabOff = 0x3ff8u;
+constexpr uint32_t kQInt = 1u << 17, kQOwn = 1u << 18, kQRow = 1u << 19;
+constexpr uint32_t kTabOff = 0x3ff8u, kTab4Off = 0x7ff0u;  // table byte offsets: 8-byte / 16-byte entries
 
-__device__ __forceinline__ float2 tab_at(const float2 *tab, uint32_t byte_off) {
-  return *reinterpret_cast<const float2 *>(reinterpret_cast<const char *>(tab) + byte_off);
+__device__ __forceinline__ void sweep_table4(float4 *tab, const SGroup &G) {
+  const int n = (G.pad + 1) * (G.pad + 1);
+  for (int k = threadIdx.x; k < n; k += kThreads) {
+    const int dz = k / (G.pad + 1), dx = k - dz * (G.pad + 1);
+    const int c = (G.pad - dz) * G.W + (G.pad - dx);
+    tab[k] = table4_entry(G.A[c], G.B[c]);
+  }
 }
 
 template <int KP>
@@ -402,6 +409,10 @@ __global__ __launch_bounds__(kThreads) void sweep_fwd_pair_kernel(SArgs a) {
 
   const uint32_t magic = 0xffffffffu / (uint32_t)EW + 1u;
   const int r = pad + G.row, sxc = pad + G.src[v.s];
+  // the source cell's pair in the extended region (every item that computes the cell injects), uniform
+  const int se = (r - v.ey0) * EW + (sxc - v.ex0);
+  const bool src_in = r >= v.ey0 && r < v.ey0 + v.EH && sxc >= v.ex0 && sxc < v.ex0 + EW;
+  const int src_j = src_in ? (se >> 1) / kThreads : -1, src_t = (se >> 1) % kThreads, src_hi = se & 1;
   const float2 *c2b = reinterpret_cast<const float2 *>(G.c2 + (int64_t)v.b * HW);
   float2 pn[KP], cc[KP];
   uint32_t da[KP], db[KP];
@@ -418,8 +429,6 @@ __global__ __launch_bounds__(kThreads) void sweep_fwd_pair_kernel(SArgs a) {
       uint32_t d = (uint32_t)gi;
       if (z >= 2 && z < H - 2 && x >= 2 && x < W - 2) d |= kQInt;
       if (z >= v.y0 && z < v.y1 && x >= v.x0 && x < v.x1) d |= kQOwn;
-      if (z == r && x == sxc) d |= kQSrc0;
-      if (z == r && x + 1 == sxc) d |= kQSrc1;
       da[j] = d;
       const int sz = sponge_dist(z, H, pad) * (pad + 1);
       db[j] = ((uint32_t)(sz + sponge_dist(x, W, pad)) << 3) | ((uint32_t)(sz + sponge_dist(x + 1, W, pad)) << 19);
@@ -456,16 +465,17 @@ __global__ __launch_bounds__(kThreads) void sweep_fwd_pair_kernel(SArgs a) {
         const float2 um = C2[p - EW2], u2m = C2[p - EW], up = C2[p + EW2], u2p = C2[p + EW];
         const float s0 = lap_sum(u2m.x, um.x, r1.x, up.x, u2p.x, r0.x, r0.y, r1.y, r2.x);
         const float s1 = lap_sum(u2m.y, um.y, r1.y, up.y, u2p.y, r0.y, r1.x, r2.x, r2.y);
-        const bool in = (d & kQInt) != 0;
         const float2 Lq = div12x2(s0, s1);
-        const float L0 = in ? Lq.x : 0.0f;
-        const float L1 = in ? Lq.y : 0.0f;
-        const float2 ab0 = tab_at(tab, db[j] & kTabOff), ab1 = tab_at(tab, (db[j] >> 16) & kTabOff);
+        const float L0 = keep_if<17>(Lq.x, d), L1 = keep_if<17>(Lq.y, d);  // interior (kQInt)
+        const float2 ab0 = lds_table2(db[j] & kTabOff), ab1 = lds_table2((db[j] >> 16) & kTabOff);
         float q0 = leapfrog(r1.x, pn[j].x, ab0.y, cc[j].x, L0, ab0.x);
         float q1 = leapfrog(r1.y, pn[j].y, ab1.y, cc[j].y, L1, ab1.x);
-        const float s0w = q0 + wsrc, s1w = q1 + wsrc;
-        q0 = (d & kQSrc0) ? s0w : q0;
-        q1 = (d & kQSrc1) ? s1w : q1;
+        if (j == src_j) {  // uniform: only the source's pair slot carries this code
+          if (t == src_t) {
+            if (src_hi) q1 = q1 + wsrc;
+            else q0 = q0 + wsrc;
+          }
+        }
         buf_store2(make_float2(L0, L1), hrs, (d & kQOwn) ? (int)(d & kOffMask) * 4 : kNoAccess, it * HW * 4);
         N2[p] = make_float2(q0, q1);
         pn[j] = r1;
@@ -505,15 +515,19 @@ __global__ __launch_bounds__(kThreads) void sweep_adj_pair_kernel(SArgs a) {
   float2 *m1o = reinterpret_cast<float2 *>(G.st + 2 * (ib ^ 1) * PS + (int64_t)v.prop * HW);
   float2 *m2o = reinterpret_cast<float2 *>(G.st + (2 * (ib ^ 1) + 1) * PS + (int64_t)v.prop * HW);
   float *Qc = lds + a.plane0, *Qn = lds + a.plane1;
-  float2 *tab = reinterpret_cast<float2 *>(lds + a.tab_off);
+  float4 *tab = reinterpret_cast<float4 *>(lds + a.tab_off);
   float2 *il = reinterpret_cast<float2 *>(lds + a.il_off);
   const int t = threadIdx.x;
   zero_guards(Qc, Qn, 2 * NPE, EW);
-  sweep_table(tab, G, 1);
+  sweep_table4(tab, G);
   __syncthreads();
 
   const uint32_t magic = 0xffffffffu / (uint32_t)EW + 1u;
   const int r = pad + G.row, sxc = pad + G.src[v.s];
+  // the source cell's pair when the source lies in the own tile (its owner sums the source gradient), uniform
+  const int se = (r - v.ey0) * EW + (sxc - v.ex0);
+  const bool src_own = r >= v.y0 && r < v.y1 && sxc >= v.x0 && sxc < v.x1;
+  const int src_j = src_own ? (se >> 1) / kThreads : -1, src_t = (se >> 1) % kThreads, src_hi = se & 1;
   const float2 *c2b = reinterpret_cast<const float2 *>(G.c2 + (int64_t)v.b * HW);
   float2 *gac = reinterpret_cast<float2 *>(G.gacc + (int64_t)v.prop * HW);
   float2 *ilg = reinterpret_cast<float2 *>(G.illum + (int64_t)v.prop * G.nz * G.nx);
@@ -537,16 +551,13 @@ __global__ __launch_bounds__(kThreads) void sweep_adj_pair_kernel(SArgs a) {
       if (interior) d |= kQInt;
       if (own) d |= kQOwn;
       if (z == r && x >= pad && x < pad + G.nx) d |= kQRow;
-      if (own && z == r && x == sxc) d |= kQSrc0;
-      if (own && z == r && x + 1 == sxc) d |= kQSrc1;
-      if (d & (kQSrc0 | kQSrc1)) sg = a.k > 0 ? G.sgrad[v.prop] : 0.0f;
       da[j] = d;
       const int sz = sponge_dist(z, H, pad) * (pad + 1);
-      const uint32_t o0 = (uint32_t)(sz + sponge_dist(x, W, pad)) << 3;
-      const uint32_t o1 = (uint32_t)(sz + sponge_dist(x + 1, W, pad)) << 3;
+      const uint32_t o0 = (uint32_t)(sz + sponge_dist(x, W, pad)) << 4;
+      const uint32_t o1 = (uint32_t)(sz + sponge_dist(x + 1, W, pad)) << 4;
       db[j] = o0 | (o1 << 16);
       const float2 cv = c2b[gi >> 1];
-      ca[j] = make_float2(cv.x * tab_at(tab, o0).x, cv.y * tab_at(tab, o1).x);  // c2 * A (gpu.py:179)
+      ca[j] = make_float2(cv.x * lds_table_a(o0), cv.y * lds_table_a(o1));  // c2 * A (gpu.py:179)
       float2 ilv = make_float2(0.0f, 0.0f);
       if (a.k > 0) {
         m1[j] = m1i[gi >> 1];
@@ -559,6 +570,7 @@ __global__ __launch_bounds__(kThreads) void sweep_adj_pair_kernel(SArgs a) {
                                                        interior ? div12(ca[j].y * m1[j].y) : 0.0f);
     }
   }
+  if (src_j >= 0 && t == src_t) sg = a.k > 0 ? G.sgrad[v.prop] : 0.0f;
   const __amdgpu_buffer_rsrc_t hrs = make_rsrc(G.hist + (int64_t)v.prop * G.nt * HW, (int64_t)G.nt * HW);
   const __amdgpu_buffer_rsrc_t rrs = make_rsrc(G.res + (int64_t)v.prop * G.nt * G.nx, (int64_t)G.nt * G.nx);
   const int rec0 = r * W + pad;
@@ -594,31 +606,30 @@ __global__ __launch_bounds__(kThreads) void sweep_adj_pair_kernel(SArgs a) {
         const float2 um = C2[p - EW2], u2m = C2[p - EW], up = C2[p + EW2], u2p = C2[p + EW];
         const float lt0 = lapt_sum(r1.x, u2p.x, up.x, um.x, u2m.x, r2.x, r1.y, r0.y, r0.x);
         const float lt1 = lapt_sum(r1.y, u2p.y, up.y, um.y, u2m.y, r2.y, r2.x, r1.x, r0.y);
-        const float2 aab0 = tab_at(tab, db[j] & kTabOff), aab1 = tab_at(tab, (db[j] >> 16) & kTabOff);
-        float mu0 = (2.0f * aab0.x) * m1[j].x;
-        float mu1 = (2.0f * aab1.x) * m1[j].y;
+        const float3 aab0 = lds_table3(db[j] & kTab4Off), aab1 = lds_table3((db[j] >> 16) & kTab4Off);  // (A, 2A, A B)
+        float mu0 = aab0.y * m1[j].x;
+        float mu1 = aab1.y * m1[j].y;
         mu0 = mu0 + lt0;
         mu1 = mu1 + lt1;
-        mu0 = mu0 - aab0.y * m2[j].x;
-        mu1 = mu1 - aab1.y * m2[j].y;
+        mu0 = mu0 - aab0.z * m2[j].x;
+        mu1 = mu1 - aab1.z * m2[j].y;
         if (d & kQRow) {
           const float2 rv = buf_load2(rrs, (gi - rec0) * 4, it * G.nx * 4);
           mu0 = mu0 + rv.x;
           mu1 = mu1 + rv.y;
         }
-        const float g0 = ga[j].x + (aab0.x * mu0) * L.x;
-        const float g1 = ga[j].y + (aab1.x * mu1) * L.y;
-        const bool own = (d & kQOwn) != 0;
-        ga[j] = make_float2(own ? g0 : ga[j].x, own ? g1 : ga[j].y);
+        // off the own tile L = 0: the imaging sum there is never stored, the illumination unchanged
+        ga[j].x = ga[j].x + (aab0.x * mu0) * L.x;
+        ga[j].y = ga[j].y + (aab1.x * mu1) * L.y;
         const float2 iv = il[p];
-        il[p] = make_float2(iv.x + L.x * L.x, iv.y + L.y * L.y);  // unchanged off the own tile (L = 0)
-        if (d & kQSrc0) sg = sg + (mu0 * wt) * sdv;
-        if (d & kQSrc1) sg = sg + (mu1 * wt) * sdv;
+        il[p] = make_float2(iv.x + L.x * L.x, iv.y + L.y * L.y);
+        if (j == src_j) {  // uniform: only the source's pair slot carries this code
+          if (t == src_t) sg = sg + ((src_hi ? mu1 : mu0) * wt) * sdv;
+        }
         m2[j] = m1[j];
         m1[j] = make_float2(mu0, mu1);
-        const bool in = (d & kQInt) != 0;
         const float2 qq = div12x2(ca[j].x * mu0, ca[j].y * mu1);
-        N2[p] = make_float2(in ? qq.x : 0.0f, in ? qq.y : 0.0f);
+        N2[p] = make_float2(keep_if<17>(qq.x, d), keep_if<17>(qq.y, d));  // interior (kQInt)
       }
     }
     __syncthreads();
@@ -638,7 +649,7 @@ __global__ __launch_bounds__(kThreads) void sweep_adj_pair_kernel(SArgs a) {
       gac[gi >> 1] = ga[j];
       const int z = (int)__umulhi((uint32_t)gi, magic_w), x = gi - z * W;
       if (z >= pad && z < pad + G.nz && x >= pad && x < pad + G.nx) ilg[((z - pad) * G.nx + (x - pad)) >> 1] = il[p];
-      if (d & (kQSrc0 | kQSrc1)) G.sgrad[v.prop] = sg;
+      if (j == src_j && t == src_t) G.sgrad[v.prop] = sg;
     }
   }
 }
@@ -667,13 +678,15 @@ int ext_cells(int H, int W, const TileChoice &c) {
 }
 
 // LDS floats for the largest extended region ehw (widest ew) and sponge width pad.
-int lds_floats(int ehw, int ew, int pad, int dir, int *plane1, int *tab_off, int *il_off) {
+// The table comes first (LDS address 0); an adjoint reserves 16-byte entries.
+int lds_floats(int ehw, int ew, int pad, int dir, int *plane0, int *plane1, int *il_off) {
   const int guard = 2 * ew + 2;
-  const int p1 = guard + ehw + guard;
-  const int tab = (p1 + ehw + guard + 1) & ~1;
-  const int il = tab + 2 * (pad + 1) * (pad + 1);
+  const int tab = (dir ? 4 : 2) * (pad + 1) * (pad + 1);
+  const int p0 = tab + guard;
+  const int p1 = p0 + ehw + guard;
+  const int il = p1 + ehw + guard;
+  if (plane0) *plane0 = p0;
   if (plane1) *plane1 = p1;
-  if (tab_off) *tab_off = tab;
   if (il_off) *il_off = il;
   return il + (dir ? ehw : 0);
 }
@@ -852,15 +865,15 @@ int make_plan(int ngroups, const ss_sweep_group *groups, int dir, Plan *p) {
   for (int k = 0; k < p->nlaunch; ++k)
     for (int i = 0; i < ngroups; ++i)
       if (gs[i].nlaunch > k) p->nactive[k] = gs[i].item0 + gs[i].nitems;
-  int plane1, tab_off, il_off;
-  const int nlds = lds_floats(ehw, ew, pad, dir, &plane1, &tab_off, &il_off);
+  int plane0, plane1, il_off;
+  const int nlds = lds_floats(ehw, ew, pad, dir, &plane0, &plane1, &il_off);
   if (nlds > kMaxLdsFloats) {
     set_error("sweep: the groups' tilings together need more than 160 KiB of LDS");
     return SS_EUNSUP;
   }
-  p->args.plane0 = 2 * ew + 2;
+  p->args.plane0 = plane0;
   p->args.plane1 = plane1;
-  p->args.tab_off = tab_off;
+  p->args.tab_off = 0;
   p->args.il_off = il_off;
   p->lds = (size_t)nlds * sizeof(float);
   // cell pairs when every plane width, pad and model width is even and every tile width is even or whole
@@ -912,6 +925,14 @@ int prepare_kernel(const Plan &p, KernelFn *fn) {
   if (e != hipSuccess) return hip_status(e, "sweep device query");
   if ((int64_t)p.lds > maxlds) {
     set_error("sweep: tiles need more LDS than the device has");
+    return SS_EUNSUP;
+  }
+  // the sponge table is addressed at LDS address 0: no static LDS may precede the dynamic LDS
+  hipFuncAttributes attr;
+  e = hipFuncGetAttributes(&attr, (const void *)*fn);
+  if (e != hipSuccess) return hip_status(e, "sweep attributes");
+  if (attr.sharedSizeBytes != 0) {
+    set_error("sweep: kernel has static LDS ahead of its sponge table");
     return SS_EUNSUP;
   }
   return hip_status(hipFuncSetAttribute((const void *)*fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)p.lds),
