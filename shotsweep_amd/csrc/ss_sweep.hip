@@ -375,7 +375,7 @@ __global__ __launch_bounds__(kThreads) void sweep_adj_kernel(SArgs a) {
 // words per pair: A = plane offset of the first cell | flags (one source flag
 // per cell), B = the two cells' sponge-table byte offsets.  Per cell the
 // arithmetic is that of the cell kernels above.
-constexpr uint32_t kQInt = 1u << 17, kQOwn = 1u << 18, kQRow = 1u << 19;
+constexpr uint32_t kQInt = 1u << 17, kQOwn = 1u << 18, kQRow = 1u << 19, kQOwnModel = 1u << 20;
 constexpr uint32_t kTabOff = 0x3ff8u, kTab4Off = 0x7ff0u;  // table byte offsets: 8-byte / 16-byte entries
 
 __device__ __forceinline__ void sweep_table4(float4 *tab, const SGroup &G) {
@@ -550,6 +550,7 @@ __global__ __launch_bounds__(kThreads) void sweep_adj_pair_kernel(SArgs a) {
       const bool model = z >= pad && z < pad + G.nz && x >= pad && x < pad + G.nx;
       if (interior) d |= kQInt;
       if (own) d |= kQOwn;
+      if (own && model) d |= kQOwnModel;
       if (z == r && x >= pad && x < pad + G.nx) d |= kQRow;
       da[j] = d;
       const int sz = sponge_dist(z, H, pad) * (pad + 1);
@@ -621,8 +622,10 @@ __global__ __launch_bounds__(kThreads) void sweep_adj_pair_kernel(SArgs a) {
         // off the own tile L = 0: the imaging sum there is never stored, the illumination unchanged
         ga[j].x = ga[j].x + (aab0.x * mu0) * L.x;
         ga[j].y = ga[j].y + (aab1.x * mu1) * L.y;
-        const float2 iv = il[p];
-        il[p] = make_float2(iv.x + L.x * L.x, iv.y + L.y * L.y);
+        if (d & kQOwnModel) {  // illumination of the own model cells only (the rest is never stored)
+          const float2 iv = il[p];
+          il[p] = make_float2(iv.x + L.x * L.x, iv.y + L.y * L.y);
+        }
         if (j == src_j) {  // uniform: only the source's pair slot carries this code
           if (t == src_t) sg = sg + ((src_hi ? mu1 : mu0) * wt) * sdv;
         }
