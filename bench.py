@@ -251,6 +251,27 @@ def copy_probe(nbytes=1 << 30, reps=3):
     print(json.dumps({'copy_probe_bytes': nbytes, 'reps': reps}), flush=True)
 
 
+def bandwidth_probe(nbytes=8 << 30, reps=5):
+    """Device fill (write only), copy (read + write) and sum (read only) rates of
+    an nbytes float32 buffer, timed with events: reference points for the
+    history stream of the FD kernels."""
+    x = torch.empty(nbytes // 4, dtype=torch.float32, device='cuda')
+    y = torch.empty_like(x)
+    out = {}
+    for name, fn, moved in (('fill', lambda: x.fill_(1.0), nbytes), ('copy', lambda: y.copy_(x), 2 * nbytes),
+                            ('sum', lambda: x.sum(), nbytes)):
+        fn()
+        torch.cuda.synchronize()
+        t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t0.record()
+        for _ in range(reps):
+            fn()
+        t1.record()
+        torch.cuda.synchronize()
+        out[name + '_TBps'] = moved * reps / (t0.elapsed_time(t1) * 1e-3) / 1e12
+    print(json.dumps({'bandwidth_probe_bytes': nbytes, **out}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -267,9 +288,12 @@ def main():
                          'autograd: the differentiation workload')
     ap.add_argument('--models-per-size', type=int, default=32)
     ap.add_argument('--copy-probe', action='store_true', help='only run 1 GiB device copies (counter scaling)')
+    ap.add_argument('--bandwidth-probe', action='store_true', help='only time 8 GiB device fill / copy / sum')
     args = ap.parse_args()
     if args.copy_probe:
         return copy_probe()
+    if args.bandwidth_probe:
+        return bandwidth_probe()
     if args.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) != 1:
         print('bench.py measures one GPU: the path shards by model with no exchange in the time loop; '
               'multi-GPU runs are not part of this build', file=sys.stderr)
