@@ -219,7 +219,8 @@ def run_autograd(args):
         'roofline': {'bound': 'hbm', 'kernel': dominant, 'achieved': alg / (per[dominant] * 1e-3) / 1e9,
                      'peak': PEAK_HBM_GBS, 'unit': 'GB/s', 'frac': alg / (per[dominant] * 1e-3) / 1e9 / PEAK_HBM_GBS,
                      'traffic': None, 'kernel_ms_per_step': per, 'algorithmic_bytes_per_direction': alg,
-                     'note': 'forward time includes the per-segment recompute (two forward passes)'},
+                     'note': 'forward time includes the per-segment recompute (two forward passes), which '
+                             'overlaps the adjoint on a side stream'},
     }
     if not args.no_cpu:
         from oracle import fd

@@ -137,6 +137,7 @@ class Sim:
         self.si = torch.arange(g.ns, device=self.device)
         self.sx = torch.tensor(list(g.src), device=self.device) + pad
         self._ws = {}
+        self._side = None    # side stream of the checkpointed adjoint (created on first use)
         self.events = None   # list -> (name, start, end) HIP events around each time-loop launch
 
     # ---------------------------------------------------------------- setup
@@ -287,9 +288,11 @@ class Sim:
         plane = g.ns * self.H * self.W * itemsize
         seg = self._segment(nb, history)
         traces = g.ns * g.nt * g.nx * itemsize
-        hist = plane * seg
-        ckpt = 0 if seg >= g.nt else plane * 2 * (-(-g.nt // seg))
-        return nb * (hist + ckpt + 5 * plane + 2 * traces)
+        if seg >= g.nt:     # whole history; p0, p1, m1, m2 and the imaging sum
+            return nb * (plane * seg + 5 * plane + 2 * traces)
+        # checkpoints, two alternating segment buffers with their own p0, p1
+        ckpt = plane * 2 * (-(-g.nt // seg))
+        return nb * (ckpt + 2 * plane * seg + 9 * plane + 2 * traces)
 
     def _segment(self, nb, history=None):
         """Steps per adjoint segment: nt (whole history stored) or the checkpoint interval."""
@@ -364,23 +367,67 @@ class Sim:
         if 'hist' in tape:
             self._adjoint(geom, 0, g.nt, c2, sd, res, tape['hist'], m1, m2, gacc, illum, sg)
         else:
-            ck = tape['ckpt']
-            p0 = self._scratch('p0', (nb, g.ns, H, W))
-            p1 = self._scratch('p1', (nb, g.ns, H, W))
-            hist = self._scratch('hist_seg', (nb, g.ns, seg, H, W))
-            for k in reversed(range(ck.shape[2])):
-                t0, t1 = k * seg, min(g.nt, (k + 1) * seg)
-                p0.copy_(ck[:, :, k, 0])
-                p1.copy_(ck[:, :, k, 1])
-                h = hist.reshape(-1)[:nb * g.ns * (t1 - t0) * H * W].view(nb, g.ns, t1 - t0, H, W)
-                self._forward(geom, t0, t1, c2, amp, p0, p1, hist=h)
-                self._adjoint(geom, t0, t1, c2, sd, res, h, m1, m2, gacc, illum, sg)
+            self._segments(geom, prep, tape, res, m1, m2, gacc, illum, sg)
         grad = torch.empty(nb, g.ns, g.nz, g.nx, dtype=self.dtype, device=self.device)
         k = np.dtype(np.float32 if self.dtype == torch.float32 else np.float64).type((g.dt / g.dx) ** 2)
         f = _lib.fn('gradient', self.dtype)
         _lib.check(f(geom, _lib.ptr(gacc), _lib.ptr(prep['v']), _lib.ptr(sg), float(k), _lib.ptr(grad),
                      self._stream()), 'gradient')
         return grad, illum
+
+    def _segments(self, geom, prep, tape, res, m1, m2, gacc, illum, sg):
+        """Checkpointed adjoint (gpu.py:169-184): for each segment, last first,
+        recompute its forward from the checkpoint, keeping the Laplacians, then
+        run the adjoint over it.  The recompute of segment k-1 runs on a side
+        stream while the adjoint of segment k runs on the caller's stream (they
+        touch different buffers; two segment buffers alternate), so a batch
+        that leaves CUs idle overlaps the two.  Each launch computes exactly
+        what it computes in sequence, so results do not depend on the overlap."""
+        g = self.geom
+        nb, c2, amp, sd = prep['nb'], prep['c2'], prep['amp'], prep['sd']
+        H, W = self.H, self.W
+        ck, seg = tape['ckpt'], tape['seg']
+        nseg = ck.shape[2]
+        main = torch.cuda.current_stream(self.device)
+        if self._side is None:
+            self._side = torch.cuda.Stream(device=self.device)  # default priority
+        side = self._side
+        bufs = [(self._scratch(f'p0_{i}', (nb, g.ns, H, W)), self._scratch(f'p1_{i}', (nb, g.ns, H, W)),
+                 self._scratch(f'hist_seg{i}', (nb, g.ns, seg, H, W))) for i in (0, 1)]
+        freed = [None, None]        # event after the adjoint that last read segment buffer i
+        ready = {}
+
+        def span(k):
+            return k * seg, min(g.nt, (k + 1) * seg)
+
+        def view(i, k):
+            t0, t1 = span(k)
+            return bufs[i][2].reshape(-1)[:nb * g.ns * (t1 - t0) * H * W].view(nb, g.ns, t1 - t0, H, W)
+
+        def recompute(k):
+            i = k % 2
+            t0, t1 = span(k)
+            if freed[i] is not None:
+                side.wait_event(freed[i])
+            with torch.cuda.stream(side):
+                p0, p1, _ = bufs[i]
+                p0.copy_(ck[:, :, k, 0])
+                p1.copy_(ck[:, :, k, 1])
+                self._forward(geom, t0, t1, c2, amp, p0, p1, hist=view(i, k))
+                ready[k] = torch.cuda.Event()
+                ready[k].record(side)
+
+        side.wait_stream(main)      # checkpoints, residual and zeroed state are ready
+        recompute(nseg - 1)
+        for k in reversed(range(nseg)):
+            if k > 0:
+                recompute(k - 1)
+            main.wait_event(ready.pop(k))
+            t0, t1 = span(k)
+            self._adjoint(geom, t0, t1, c2, sd, res, view(k % 2, k), m1, m2, gacc, illum, sg)
+            freed[k % 2] = torch.cuda.Event()
+            freed[k % 2].record(main)
+        main.wait_stream(side)      # later allocations on the caller's stream come after the side work
 
     def _gradient_device(self, prep, observed=None, obs_repeat=1, v_mute=None, t_mute=0.12, res=None):
         """Forward, muted residual, exact adjoint and fold on the device.
