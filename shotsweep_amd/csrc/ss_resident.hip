@@ -397,7 +397,7 @@ __global__ __launch_bounds__(kFwdThreads) void fwd_pair_kernel(Geom g, int t0, i
     int off = t;
 #pragma unroll
     for (int j = 0; j < KP; ++j) {
-      asm volatile("" : "+v"(off), "+v"(pk[j]));  // one pair at a time
+      asm volatile("" : "+v"(pk[j]));  // one pair at a time
       if (t + j * kFwdThreads < NP) {
         const int q = off + j * kFwdThreads;
         const uint32_t d0 = pk[j] & 0xffffu, d1 = pk[j] >> 16;
@@ -753,7 +753,7 @@ __global__ __launch_bounds__(kAdjThreads) void adj_pair_kernel(Geom g, int t0, i
     opaque(pk);
 #pragma unroll
     for (int j = 0; j < KP; ++j) {
-      asm volatile("" : "+v"(off), "+v"(pk[j]));
+      asm volatile("" : "+v"(pk[j]));
       const float2 cv = Cb[j % D];
       {
         // c2 is the same every step: after the last pair, load the first pairs of the next step
