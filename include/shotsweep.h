@@ -71,6 +71,18 @@ int ss_version(void);
 int64_t ss_resident_max_cells(void);
 
 /*
+ * Device bytes the caller allocates for one gradient or images call on g
+ * (the forward with its tape, residual, adjoint and fold), with elements of
+ * elem_size bytes (4 or 8): segment >= nt keeps the whole Laplacian history;
+ * a smaller segment keeps checkpoints every `segment` steps plus two
+ * alternating segment buffers.  This implementation's counterpart of
+ * gpu.py:70-73 (imaging_memory_bytes), which estimates the reference's own
+ * scheme.  Returns -1 (ss_last_error set) on an invalid geometry or size.
+ * Host-only arithmetic: no device is touched.
+ */
+int64_t ss_workspace_bytes(const ss_geom *g, int segment, int elem_size);
+
+/*
  * Forward time loop over steps [t0, t1), replacing gpu.py:111-115 (Sim._step)
  * and the loop gpu.py:126-135 (Sim.forward), including lap (gpu.py:33-37).
  *

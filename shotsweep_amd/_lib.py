@@ -11,7 +11,7 @@ import torch
 LIB_PATH = Path(__file__).resolve().parent / 'libshotsweep.so'
 MAX_SHOTS = 32
 IMPL_AUTO, IMPL_STEPWISE, IMPL_RESIDENT = 0, 1, 2
-ABI_VERSION = 3
+ABI_VERSION = 4
 
 _vp = ctypes.c_void_p
 _i = ctypes.c_int
@@ -88,10 +88,18 @@ _QC = {
     'ss_velocity_scores_f64': [_i, _i, _i, _vp, _vp, _d, _d, _vp, _vp, _vp],
 }
 
-EXPORTS = ['ss_last_error', 'ss_version', 'ss_resident_max_cells'] + [
+EXPORTS = ['ss_last_error', 'ss_version', 'ss_resident_max_cells', 'ss_workspace_bytes'] + [
     f'ss_{name}_{sfx}' for sfx in ('f32', 'f64') for name in _sig(ctypes.c_float)] + list(_CNN) + list(_SWEEP) + list(_QC)
 
 _LIB = None
+
+
+def workspace_bytes(geom, segment, elem_size):
+    """ss_workspace_bytes: device bytes of one gradient/images call (host arithmetic only)."""
+    n = load().ss_workspace_bytes(ctypes.byref(geom), int(segment), int(elem_size))
+    if n < 0:
+        raise ValueError(load().ss_last_error().decode())
+    return int(n)
 
 
 def load():
@@ -107,6 +115,8 @@ def load():
     lib.ss_last_error.argtypes = []
     lib.ss_version.restype = ctypes.c_int
     lib.ss_resident_max_cells.restype = ctypes.c_int64
+    lib.ss_workspace_bytes.restype = ctypes.c_int64
+    lib.ss_workspace_bytes.argtypes = [ctypes.POINTER(SSGeom), ctypes.c_int, ctypes.c_int]
     for sfx, real in (('f32', ctypes.c_float), ('f64', ctypes.c_double)):
         for name, args in _sig(real).items():
             fn = getattr(lib, f'ss_{name}_{sfx}')

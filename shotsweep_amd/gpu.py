@@ -283,16 +283,8 @@ class Sim:
     # --------------------------------------------------------- memory plan
     def workspace_bytes(self, nb, history=None):
         """Device bytes of one gradient/images call on nb rows under a history mode."""
-        g = self.geom
         itemsize = torch.empty((), dtype=self.dtype).element_size()
-        plane = g.ns * self.H * self.W * itemsize
-        seg = self._segment(nb, history)
-        traces = g.ns * g.nt * g.nx * itemsize
-        if seg >= g.nt:     # whole history; p0, p1, m1, m2 and the imaging sum
-            return nb * (plane * seg + 5 * plane + 2 * traces)
-        # checkpoints, two alternating segment buffers with their own p0, p1
-        ckpt = plane * 2 * (-(-g.nt // seg))
-        return nb * (ckpt + 2 * plane * seg + 9 * plane + 2 * traces)
+        return _lib.workspace_bytes(self._geom(nb), self._segment(nb, history), itemsize)
 
     def _segment(self, nb, history=None):
         """Steps per adjoint segment: nt (whole history stored) or the checkpoint interval."""

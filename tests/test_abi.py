@@ -80,3 +80,22 @@ def test_sweep_validation_without_gpu_work():
     rc = lib.ss_sweep_forward_f32(1, arr, 0, None)
     assert rc == -3 and b'cells per workgroup' in lib.ss_last_error()
     assert lib.ss_sweep_autotune_f32(1, arr, 2, 0, None) == -1
+
+
+def test_workspace_bytes_closed_form():
+    """ss_workspace_bytes (host arithmetic): whole history = history planes + 5
+    state planes + 2 trace sets per row; checkpoints every s steps = 2 ceil(nt/s)
+    checkpoint planes + 2 segment buffers of s planes + 9 state planes + 2 trace
+    sets.  The reference's own estimate (gpu.py:70-73) is a different scheme."""
+    g = _lib.make_geom(192, 5, 70, 70, 40, 1000, 1, (0, 17, 34, 52, 69))
+    plane, traces = 5 * 150 * 150 * 4, 5 * 1000 * 70 * 4
+    assert _lib.workspace_bytes(g, 1000, 4) == 192 * (1000 * plane + 5 * plane + 2 * traces)
+    assert _lib.workspace_bytes(g, 5000, 4) == _lib.workspace_bytes(g, 1000, 4)
+    assert _lib.workspace_bytes(g, 50, 4) == 192 * (40 * plane + 100 * plane + 9 * plane + 2 * traces)
+    assert _lib.workspace_bytes(g, 73, 8) == 2 * 192 * (2 * 14 * plane + 2 * 73 * plane + 9 * plane + 2 * traces)
+    # 86.4 GB of Laplacian history for the 960 shot-images of the batched workload
+    assert abs(_lib.workspace_bytes(g, 1000, 4) / 1e9 - 87.0) < 0.5
+    with pytest.raises(ValueError, match='segment must be positive'):
+        _lib.workspace_bytes(g, 0, 4)
+    with pytest.raises(ValueError, match='geometry'):
+        _lib.workspace_bytes(_lib.make_geom(0, 5, 70, 70, 40, 1000, 1, (0, 17, 34, 52, 69)), 50, 4)
