@@ -14,7 +14,7 @@
 // table; each thread keeps m1, m2, the imaging accumulator and the packed
 // descriptors of its cells in registers (512 threads, 44 cells each), streams
 // the forward Laplacian from HBM a few cells ahead of use, and re-reads c2
-// from the L2-resident c2 plane when it publishes q.
+// from the L2-resident c2 plane, also a few cells ahead, when it publishes q.
 //
 // Cell ownership: cell i of the H*W plane belongs to thread i % threads as its
 // slot j = i / threads, so every LDS and HBM access of a wavefront touches 64
@@ -28,8 +28,9 @@
 // with its shape as constants (fwd_pair_kernel, adj_pair_kernel): a thread
 // owns cell pairs (2q, 2q+1), reads both cells' taps with 8-byte LDS accesses
 // and moves history, c2 and residual two cells per buffer instruction; their
-// descriptors hold the sponge-table byte offset directly.  Other shapes run
-// the cell kernels with runtime dimensions.
+// descriptors hold the sponge-table byte offset directly, and the table sits
+// at LDS address 0 ((A, B) entries for the forward, (A, 2A, A*B, 0) for the
+// adjoint).  Other shapes run the cell kernels with runtime dimensions.
 #include <algorithm>
 
 #include "ss_device.h"
