@@ -44,6 +44,24 @@ def measured_traffic(kernel):
         return None
 
 
+def measured_lds(kernel, kernel_ms):
+    """LDS bytes per launch of a kernel of the batched workload (rocprofv3
+    SQ_INSTS_LDS_LOAD_BANDWIDTH + SQ_INSTS_LDS_STORE_BANDWIDTH, 64-byte units)
+    and its issue figures, from profiles/lds_valu.json, as an LDS roofline
+    beside the HBM one; None when not recorded."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'profiles', 'lds_valu.json')
+    try:
+        with open(path) as f:
+            rec = json.load(f)
+        k = rec['kernels'][kernel]
+        achieved = k['lds_bytes'] / (kernel_ms * 1e-3) / 1e9
+        return {'bytes': k['lds_bytes'], 'achieved': achieved, 'peak': rec['peak_lds_GBps'], 'unit': 'GB/s',
+                'frac': achieved / rec['peak_lds_GBps'], 'valu_busy': k['valu_busy'], 'lds_util': k['lds_util'],
+                'source': 'profiles/lds_valu.json (rocprofv3 LDS bandwidth and issue counters)'}
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def workload(n_models, seed, sim):
     """Models, observed gathers (device), backgrounds and mute velocities."""
     from shotsweep_amd import background as BG, synth
@@ -349,7 +367,8 @@ def main():
         'roofline': {'bound': 'hbm', 'kernel': dominant, 'achieved': achieved, 'peak': PEAK_HBM_GBS, 'unit': 'GB/s',
                      'frac': achieved / PEAK_HBM_GBS, 'traffic': measured_traffic(dominant),
                      'traffic_source': 'profiles/hbm_traffic.json (rocprofv3 FETCH_SIZE x2.0, WRITE_SIZE x1.0)',
-                     'kernel_ms': avg, 'algorithmic_bytes_per_launch': alg_bytes},
+                     'kernel_ms': avg, 'algorithmic_bytes_per_launch': alg_bytes,
+                     'lds': measured_lds(dominant, avg[dominant])},
     }
     # first-break picking for the surface-velocity anchors: HIP kernel vs the host NumPy picks
     from shotsweep_amd import background as BG
