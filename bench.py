@@ -214,16 +214,20 @@ def run_autograd(args):
     for _ in range(args.warmup):
         autograd.misfit_and_gradient(start, obs, sim)
     torch.cuda.synchronize()
-    sim.events = []
     t0 = time.perf_counter()
     for _ in range(args.steps):
         loss, grad = autograd.misfit_and_gradient(start, obs, sim)
     torch.cuda.synchronize()
     wall = (time.perf_counter() - t0) / args.steps
+    # kernel breakdown from one more step with HIP events around every launch (the events
+    # cost host time per launch, so the timed steps above run without them)
+    sim.events = []
+    autograd.misfit_and_gradient(start, obs, sim)
+    torch.cuda.synchronize()
     events, sim.events = sim.events, None
     per = {}
     for name, a, b in events:
-        per[name] = per.get(name, 0.0) + a.elapsed_time(b) / args.steps
+        per[name] = per.get(name, 0.0) + a.elapsed_time(b)
     dominant = max(per, key=per.get)
     units = len(true) * ns
     alg = BYTES_PER_CELL_UPDATE * sim.H * sim.W * sim.geom.nt * units
@@ -237,8 +241,8 @@ def run_autograd(args):
         'roofline': {'bound': 'hbm', 'kernel': dominant, 'achieved': alg / (per[dominant] * 1e-3) / 1e9,
                      'peak': PEAK_HBM_GBS, 'unit': 'GB/s', 'frac': alg / (per[dominant] * 1e-3) / 1e9 / PEAK_HBM_GBS,
                      'traffic': None, 'kernel_ms_per_step': per, 'algorithmic_bytes_per_direction': alg,
-                     'note': 'forward time includes the per-segment recompute (two forward passes), which '
-                             'overlaps the adjoint on a side stream'},
+                     'note': 'forward time includes the per-segment recompute (two forward passes); kernel '
+                             'times from one extra step with events around every launch'},
     }
     if not args.no_cpu:
         from oracle import fd

@@ -285,6 +285,16 @@ int ss_velocity_scores_f64(int B, int H, int W, const double *pred, const double
  *   gathers           forward output (nb, ns, nt, nx) or NULL
  *   hist              Laplacian history (nb, ns, nt, H, W): forward writes, adjoint reads
  *   gacc, illum, src_grad   adjoint outputs (nb,ns,H,W), (nb,ns,nz,nx), (nb,ns), overwritten
+ *   t0, t1            step range of this call, [t0, t1) (t1 = 0: [0, nt)); hist then holds
+ *                     (nb, ns, t1-t0, H, W) and gathers / res stay (nb, ns, nt, nx).  The
+ *                     adjoint runs its range backwards.  A checkpointed gradient calls the
+ *                     forward and the adjoint once per segment (gpu.py:169-184).
+ *   resume            0: start from zero state and overwrite the adjoint outputs;
+ *                     1: read the state from state buffer `phase` and add to the outputs
+ *   phase             launch k of the call reads state buffer (k + phase) % 2 and writes the
+ *                     other; after n launches the state is in buffer (n + phase) % 2.  The
+ *                     state buffer b holds planes (p0, p1) or (m1, m2) at
+ *                     state + (2 b + plane) * nb * ns * H * W.
  * Limits: H*W <= 131072, pad <= 44, nt*H*W < 2^29.
  * use_graph = 1 captures the launch sequence once per layout (group order,
  * tilings and pointers) and replays it; 0 launches directly.
@@ -301,6 +311,7 @@ typedef struct ss_sweep_group {
   float *gathers;
   float *hist;
   float *gacc, *illum, *src_grad;
+  int32_t t0, t1, resume, phase;
 } ss_sweep_group;
 
 int ss_sweep_forward_f32(int ngroups, const ss_sweep_group *groups, int use_graph, void *stream);

@@ -11,7 +11,7 @@ import torch
 LIB_PATH = Path(__file__).resolve().parent / 'libshotsweep.so'
 MAX_SHOTS = 32
 IMPL_AUTO, IMPL_STEPWISE, IMPL_RESIDENT = 0, 1, 2
-ABI_VERSION = 4
+ABI_VERSION = 5
 
 _vp = ctypes.c_void_p
 _i = ctypes.c_int
@@ -57,7 +57,8 @@ class SSSweepGroup(ctypes.Structure):
     """Mirror of ss_sweep_group (one grid-size group of a mixed-size sweep)."""
     _fields_ = [('geom', SSGeom), ('tile_h', ctypes.c_int32), ('tile_w', ctypes.c_int32), ('depth', ctypes.c_int32),
                 ('c2', _vp), ('A', _vp), ('B', _vp), ('w', _vp), ('amp', _vp), ('res', _vp), ('state', _vp),
-                ('gathers', _vp), ('hist', _vp), ('gacc', _vp), ('illum', _vp), ('src_grad', _vp)]
+                ('gathers', _vp), ('hist', _vp), ('gacc', _vp), ('illum', _vp), ('src_grad', _vp),
+                ('t0', ctypes.c_int32), ('t1', ctypes.c_int32), ('resume', ctypes.c_int32), ('phase', ctypes.c_int32)]
 
 
 SWEEP_MAX_GROUPS = 4

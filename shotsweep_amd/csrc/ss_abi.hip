@@ -289,7 +289,7 @@ extern "C" {
 
 const char *ss_last_error(void) { return ss::g_last_error.c_str(); }
 
-int ss_version(void) { return 4; }
+int ss_version(void) { return 5; }
 
 int64_t ss_resident_max_cells(void) { return resident_max_cells(); }
 

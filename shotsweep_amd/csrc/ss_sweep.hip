@@ -69,6 +69,8 @@ struct SGroup {
   float *gathers, *hist, *gacc, *illum, *sgrad;
   int ns, nz, nx, pad, nt, row, H, W;
   int th, tw, T, ntx, ntiles, nlaunch, item0, nitems;
+  int tlo, thi;        // step range of this call; hist holds steps [tlo, thi)
+  int resume, phase;   // resume: launch 0 reads state and sums; launch k reads state buffer (k + phase) % 2
   int src[SS_MAX_SHOTS];
 };
 
@@ -157,9 +159,9 @@ __global__ __launch_bounds__(kThreads) void sweep_fwd_kernel(SArgs a) {
   const ItemView v = item_view(G, item);
   const int H = G.H, W = G.W, pad = G.pad, HW = H * W;
   const int EW = v.EW, EHW = v.EH * v.EW;
-  const int t0 = a.k * G.T, n = min(G.T, G.nt - t0);
+  const int t0 = G.tlo + a.k * G.T, n = min(G.T, G.thi - t0);
   const int64_t PS = (int64_t)(G.nitems / G.ntiles) * HW;
-  const int ib = a.k & 1;
+  const int ib = (a.k + G.phase) & 1;
   const float *p0i = G.st + 2 * ib * PS + (int64_t)v.prop * HW, *p1i = p0i + PS;
   float *p0o = G.st + 2 * (ib ^ 1) * PS + (int64_t)v.prop * HW, *p1o = p0o + PS;
   float *Pc = lds + a.plane0, *Pn = lds + a.plane1;
@@ -189,7 +191,7 @@ __global__ __launch_bounds__(kThreads) void sweep_fwd_kernel(SArgs a) {
       if (z == r && x == sxc) d |= kSrc;
       dsc[j] = d;
       cc[j] = c2b[gi];
-      if (a.k > 0) {
+      if (a.k > 0 || G.resume) {
         pn[j] = p0i[gi];
         Pc[e] = p1i[gi];
       } else {
@@ -197,7 +199,8 @@ __global__ __launch_bounds__(kThreads) void sweep_fwd_kernel(SArgs a) {
       }
     }
   }
-  const __amdgpu_buffer_rsrc_t hrs = make_rsrc(G.hist + (int64_t)v.prop * G.nt * HW, (int64_t)G.nt * HW);
+  const int64_t hspan = G.thi - G.tlo;  // history slots of this call
+  const __amdgpu_buffer_rsrc_t hrs = make_rsrc(G.hist + (int64_t)v.prop * hspan * HW, hspan * HW);
   // own receivers: row r, model columns [xa, xb) of this tile
   const int xa = max(v.x0, pad), xb = min(v.x1, pad + G.nx);
   const bool rec = G.gathers != nullptr && r >= v.y0 && r < v.y1 && t < xb - xa;
@@ -223,7 +226,7 @@ __global__ __launch_bounds__(kThreads) void sweep_fwd_kernel(SArgs a) {
         float p2 = leapfrog(c, pn[j], ab.y, cc[j], L, ab.x);
         const float ps = p2 + wsrc;
         p2 = (d & kSrc) ? ps : p2;
-        buf_store(L, hrs, (d & kOwn) ? (int)(d & kOffMask) * 4 : kNoAccess, it * HW * 4);
+        buf_store(L, hrs, (d & kOwn) ? (int)(d & kOffMask) * 4 : kNoAccess, (it - G.tlo) * HW * 4);
         Pn[e] = p2;
         pn[j] = c;
       }
@@ -257,9 +260,9 @@ __global__ __launch_bounds__(kThreads) void sweep_adj_kernel(SArgs a) {
   const ItemView v = item_view(G, item);
   const int H = G.H, W = G.W, pad = G.pad, HW = H * W;
   const int EW = v.EW, EHW = v.EH * v.EW;
-  const int hi = G.nt - a.k * G.T, n = min(G.T, hi);
+  const int hi = G.thi - a.k * G.T, n = min(G.T, hi - G.tlo);
   const int64_t PS = (int64_t)(G.nitems / G.ntiles) * HW;
-  const int ib = a.k & 1;
+  const int ib = (a.k + G.phase) & 1;
   const float *m1i = G.st + 2 * ib * PS + (int64_t)v.prop * HW, *m2i = m1i + PS;
   float *m1o = G.st + 2 * (ib ^ 1) * PS + (int64_t)v.prop * HW, *m2o = m1o + PS;
   float *Qc = lds + a.plane0, *Qn = lds + a.plane1;
@@ -297,12 +300,12 @@ __global__ __launch_bounds__(kThreads) void sweep_adj_kernel(SArgs a) {
       if (z == r && x >= pad && x < pad + G.nx) d |= kRow;
       if (own && z == r && x == sxc) {
         d |= kSrc;
-        sg = a.k > 0 ? G.sgrad[v.prop] : 0.0f;
+        sg = (a.k > 0 || G.resume) ? G.sgrad[v.prop] : 0.0f;
       }
       dsc[j] = d;
       ca[j] = c2b[gi] * tab[ti].x;  // c2 * A (gpu.py:179)
       float ilv = 0.0f;
-      if (a.k > 0) {
+      if (a.k > 0 || G.resume) {
         m1[j] = m1i[gi];
         m2[j] = m2i[gi];
         if (own) ga[j] = gac[gi];
@@ -312,7 +315,8 @@ __global__ __launch_bounds__(kThreads) void sweep_adj_kernel(SArgs a) {
       Qc[e] = interior ? div12(ca[j] * m1[j]) : 0.0f;
     }
   }
-  const __amdgpu_buffer_rsrc_t hrs = make_rsrc(G.hist + (int64_t)v.prop * G.nt * HW, (int64_t)G.nt * HW);
+  const int64_t hspan = G.thi - G.tlo;  // history slots of this call
+  const __amdgpu_buffer_rsrc_t hrs = make_rsrc(G.hist + (int64_t)v.prop * hspan * HW, hspan * HW);
   const __amdgpu_buffer_rsrc_t rrs = make_rsrc(G.res + (int64_t)v.prop * G.nt * G.nx, (int64_t)G.nt * G.nx);
   const int rec0 = r * W + pad;
   const float sdv = G.amp[v.prop];
@@ -327,7 +331,7 @@ __global__ __launch_bounds__(kThreads) void sweep_adj_kernel(SArgs a) {
       if (e < EHW) {
         const uint32_t d = dsc[j];
         const int gi = (int)(d & kOffMask);
-        const float L = buf_load(hrs, (d & kOwn) ? gi * 4 : kNoAccess, it * HW * 4);  // 0 off the own tile
+        const float L = buf_load(hrs, (d & kOwn) ? gi * 4 : kNoAccess, (it - G.tlo) * HW * 4);  // 0 off the own tile
         const float lt = lapt_sum(Qc[e], Qc[e + 2 * EW], Qc[e + EW], Qc[e - EW], Qc[e - 2 * EW], Qc[e + 2],
                                   Qc[e + 1], Qc[e - 1], Qc[e - 2]);
         const float2 aab = tab[(d >> kTabShift) & kTabMask];  // (A, A*B)
@@ -396,9 +400,9 @@ __global__ __launch_bounds__(kThreads) void sweep_fwd_pair_kernel(SArgs a) {
   const ItemView v = item_view(G, item);
   const int H = G.H, W = G.W, pad = G.pad, HW = H * W;
   const int EW = v.EW, EW2 = EW / 2, NPE = v.EH * v.EW / 2;
-  const int t0 = a.k * G.T, n = min(G.T, G.nt - t0);
+  const int t0 = G.tlo + a.k * G.T, n = min(G.T, G.thi - t0);
   const int64_t PS = (int64_t)(G.nitems / G.ntiles) * HW;
-  const int ib = a.k & 1;
+  const int ib = (a.k + G.phase) & 1;
   const float *p0i = G.st + 2 * ib * PS + (int64_t)v.prop * HW, *p1i = p0i + PS;
   float *p0o = G.st + 2 * (ib ^ 1) * PS + (int64_t)v.prop * HW, *p1o = p0o + PS;
   float *Pc = lds + a.plane0, *Pn = lds + a.plane1;
@@ -434,7 +438,7 @@ __global__ __launch_bounds__(kThreads) void sweep_fwd_pair_kernel(SArgs a) {
       db[j] = ((uint32_t)(sz + sponge_dist(x, W, pad)) << 3) | ((uint32_t)(sz + sponge_dist(x + 1, W, pad)) << 19);
       cc[j] = c2b[gi >> 1];
       float2 *P2 = reinterpret_cast<float2 *>(Pc);
-      if (a.k > 0) {
+      if (a.k > 0 || G.resume) {
         pn[j] = reinterpret_cast<const float2 *>(p0i)[gi >> 1];
         P2[p] = reinterpret_cast<const float2 *>(p1i)[gi >> 1];
       } else {
@@ -442,7 +446,8 @@ __global__ __launch_bounds__(kThreads) void sweep_fwd_pair_kernel(SArgs a) {
       }
     }
   }
-  const __amdgpu_buffer_rsrc_t hrs = make_rsrc(G.hist + (int64_t)v.prop * G.nt * HW, (int64_t)G.nt * HW);
+  const int64_t hspan = G.thi - G.tlo;  // history slots of this call
+  const __amdgpu_buffer_rsrc_t hrs = make_rsrc(G.hist + (int64_t)v.prop * hspan * HW, hspan * HW);
   const int xa = max(v.x0, pad), xb = min(v.x1, pad + G.nx);
   const bool rec = G.gathers != nullptr && r >= v.y0 && r < v.y1 && t < xb - xa;
   const int rec_e = (r - v.ey0) * EW + (xa + t - v.ex0);
@@ -476,7 +481,7 @@ __global__ __launch_bounds__(kThreads) void sweep_fwd_pair_kernel(SArgs a) {
             else q0 = q0 + wsrc;
           }
         }
-        buf_store2(make_float2(L0, L1), hrs, (d & kQOwn) ? (int)(d & kOffMask) * 4 : kNoAccess, it * HW * 4);
+        buf_store2(make_float2(L0, L1), hrs, (d & kQOwn) ? (int)(d & kOffMask) * 4 : kNoAccess, (it - G.tlo) * HW * 4);
         N2[p] = make_float2(q0, q1);
         pn[j] = r1;
       }
@@ -507,9 +512,9 @@ __global__ __launch_bounds__(kThreads) void sweep_adj_pair_kernel(SArgs a) {
   const ItemView v = item_view(G, item);
   const int H = G.H, W = G.W, pad = G.pad, HW = H * W;
   const int EW = v.EW, EW2 = EW / 2, NPE = v.EH * v.EW / 2;
-  const int hi = G.nt - a.k * G.T, n = min(G.T, hi);
+  const int hi = G.thi - a.k * G.T, n = min(G.T, hi - G.tlo);
   const int64_t PS = (int64_t)(G.nitems / G.ntiles) * HW;
-  const int ib = a.k & 1;
+  const int ib = (a.k + G.phase) & 1;
   const float2 *m1i = reinterpret_cast<const float2 *>(G.st + 2 * ib * PS + (int64_t)v.prop * HW);
   const float2 *m2i = reinterpret_cast<const float2 *>(G.st + (2 * ib + 1) * PS + (int64_t)v.prop * HW);
   float2 *m1o = reinterpret_cast<float2 *>(G.st + 2 * (ib ^ 1) * PS + (int64_t)v.prop * HW);
@@ -560,7 +565,7 @@ __global__ __launch_bounds__(kThreads) void sweep_adj_pair_kernel(SArgs a) {
       const float2 cv = c2b[gi >> 1];
       ca[j] = make_float2(cv.x * lds_table_a(o0), cv.y * lds_table_a(o1));  // c2 * A (gpu.py:179)
       float2 ilv = make_float2(0.0f, 0.0f);
-      if (a.k > 0) {
+      if (a.k > 0 || G.resume) {
         m1[j] = m1i[gi >> 1];
         m2[j] = m2i[gi >> 1];
         if (own) ga[j] = gac[gi >> 1];
@@ -571,8 +576,9 @@ __global__ __launch_bounds__(kThreads) void sweep_adj_pair_kernel(SArgs a) {
                                                        interior ? div12(ca[j].y * m1[j].y) : 0.0f);
     }
   }
-  if (src_j >= 0 && t == src_t) sg = a.k > 0 ? G.sgrad[v.prop] : 0.0f;
-  const __amdgpu_buffer_rsrc_t hrs = make_rsrc(G.hist + (int64_t)v.prop * G.nt * HW, (int64_t)G.nt * HW);
+  if (src_j >= 0 && t == src_t) sg = (a.k > 0 || G.resume) ? G.sgrad[v.prop] : 0.0f;
+  const int64_t hspan = G.thi - G.tlo;  // history slots of this call
+  const __amdgpu_buffer_rsrc_t hrs = make_rsrc(G.hist + (int64_t)v.prop * hspan * HW, hspan * HW);
   const __amdgpu_buffer_rsrc_t rrs = make_rsrc(G.res + (int64_t)v.prop * G.nt * G.nx, (int64_t)G.nt * G.nx);
   const int rec0 = r * W + pad;
   const float sdv = G.amp[v.prop];
@@ -582,7 +588,7 @@ __global__ __launch_bounds__(kThreads) void sweep_adj_pair_kernel(SArgs a) {
   float2 Lb[D];
 #pragma unroll
   for (int j = 0; j < D; ++j)
-    Lb[j] = buf_load2(hrs, (da[j] & kQOwn) ? (int)(da[j] & kOffMask) * 4 : kNoAccess, (hi - 1) * HW * 4);
+    Lb[j] = buf_load2(hrs, (da[j] & kQOwn) ? (int)(da[j] & kOffMask) * 4 : kNoAccess, (hi - 1 - G.tlo) * HW * 4);
   __syncthreads();
 
   for (int st = 0; st < n; ++st) {
@@ -598,7 +604,7 @@ __global__ __launch_bounds__(kThreads) void sweep_adj_pair_kernel(SArgs a) {
         // wraps to the first pairs of the next step (the last step reloads its own: unused)
         const int jn = j + D < KP ? j + D : j + D - KP;
         const int itn = (j + D < KP || st + 1 == n) ? it : it - 1;
-        Lb[j % D] = buf_load2(hrs, (da[jn] & kQOwn) ? (int)(da[jn] & kOffMask) * 4 : kNoAccess, itn * HW * 4);
+        Lb[j % D] = buf_load2(hrs, (da[jn] & kQOwn) ? (int)(da[jn] & kOffMask) * 4 : kNoAccess, (itn - G.tlo) * HW * 4);
       }
       if (p < NPE) {
         const uint32_t d = da[j];
@@ -756,6 +762,10 @@ int fill_group(const ss_sweep_group &in, int dir, SGroup *o, char *err, size_t e
              "(H*W <= %d, pad <= 44, nt*H*W < 2^29)", g.H, g.W, g.pad, g.nt, kMaxPlane);
     return SS_EUNSUP;
   }
+  if (in.t0 < 0 || in.t1 < 0 || (in.t1 > 0 && (in.t1 > g.nt || in.t0 >= in.t1)) || (in.t1 == 0 && in.t0 >= g.nt)) {
+    snprintf(err, errn, "sweep: step range [%d, %d) outside [0, %d)", in.t0, in.t1, g.nt);
+    return SS_EINVAL;
+  }
   const void *need[] = {in.c2, in.A, in.B, in.w, in.amp, in.state, in.hist};
   for (const void *p : need)
     if (!p) {
@@ -823,7 +833,11 @@ int fill_group(const ss_sweep_group &in, int dir, SGroup *o, char *err, size_t e
   o->T = tc.T;
   o->ntx = ntx;
   o->ntiles = ntx * nty;
-  o->nlaunch = (g.nt + tc.T - 1) / tc.T;
+  o->tlo = in.t0;
+  o->thi = in.t1 > 0 ? in.t1 : g.nt;
+  o->resume = in.resume ? 1 : 0;
+  o->phase = in.phase & 1;
+  o->nlaunch = (o->thi - o->tlo + tc.T - 1) / tc.T;
   o->nitems = (int)(nprop * o->ntiles);
   for (int s = 0; s < SS_MAX_SHOTS; ++s) o->src[s] = g.src[s];
   return SS_OK;
@@ -1038,7 +1052,7 @@ double time_choice(const ss_sweep_group &grp, int dir, const TileChoice &tc, hip
   if (hipEventSynchronize(e1) != hipSuccess) return -1.0;
   float ms = 0.0f;
   if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess) return -1.0;
-  const double steps = (double)std::min(G.nt, m * G.T);
+  const double steps = (double)std::min(G.thi - G.tlo, m * G.T);
   return ms / (steps * (double)G.nitems / G.ntiles * G.H * G.W);
 }
 

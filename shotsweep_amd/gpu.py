@@ -107,11 +107,13 @@ class Sim:
 
     def __init__(self, wavelet=None, pad=40, R=1e-5, checkpoint_interval=40,
                  device=None, dtype=torch.float32, *, geom=None, history='auto', max_history_bytes=None,
-                 impl=_lib.IMPL_AUTO):
+                 impl=_lib.IMPL_AUTO, segment_impl='auto'):
         if pad < 2 or checkpoint_interval < 1:
             raise ValueError('pad must be >= 2 and checkpoint_interval must be positive')
         if history not in ('auto', 'full', 'checkpoint'):
             raise ValueError("history must be 'auto', 'full' or 'checkpoint'")
+        if segment_impl not in ('auto', 'resident', 'tiled'):
+            raise ValueError("segment_impl must be 'auto', 'resident' or 'tiled'")
         self.geom = OPENFWI if geom is None else geom
         self.pad = pad
         self.checkpoint_interval = checkpoint_interval
@@ -138,6 +140,8 @@ class Sim:
         self.sx = torch.tensor(list(g.src), device=self.device) + pad
         self._ws = {}
         self._side = None    # side stream of the checkpointed adjoint (created on first use)
+        self.segment_impl = segment_impl
+        self._seg_tiles = {}  # direction -> (tile_h, tile_w, depth) of the tiled segment path
         self.events = None   # list -> (name, start, end) HIP events around each time-loop launch
 
     # ---------------------------------------------------------------- setup
@@ -170,8 +174,11 @@ class Sim:
         if v.ndim != 3 or v.shape[1:] != (g.nz, g.nx) or not np.isfinite(v).all() or np.any(v <= 0):
             raise ValueError(f'velocities must be finite positive (B, {g.nz}, {g.nx}) arrays')
         p = self.pad
-        vp = np.pad(v, ((0, 0), (p, p), (p, p)), mode='edge')
-        c2 = self._tensor(vp ** 2 * (g.dt / g.dx) ** 2)[:, None]
+        # edge padding and vp^2 (dt/dx)^2 in float64 on the device (exact IEEE products, the
+        # reference's order), then one rounding to the working dtype
+        vd = torch.as_tensor(v, dtype=torch.float64, device=self.device)
+        vp = torch.nn.functional.pad(vd[:, None], (p, p, p, p), mode='replicate')
+        c2 = ((vp * vp) * ((g.dt / g.dx) ** 2)).to(self.dtype)
         amp = self._tensor(np.ascontiguousarray((v[:, g.row, list(g.src)] / 2000.0) ** 2))
         return c2, amp
 
@@ -358,6 +365,8 @@ class Sim:
         seg = tape['seg']
         if 'hist' in tape:
             self._adjoint(geom, 0, g.nt, c2, sd, res, tape['hist'], m1, m2, gacc, illum, sg)
+        elif self._tiled_segments(nb):
+            self._segments_tiled(prep, tape, res, gacc, illum, sg)
         else:
             self._segments(geom, prep, tape, res, m1, m2, gacc, illum, sg)
         grad = torch.empty(nb, g.ns, g.nz, g.nx, dtype=self.dtype, device=self.device)
@@ -420,6 +429,78 @@ class Sim:
             freed[k % 2] = torch.cuda.Event()
             freed[k % 2].record(main)
         main.wait_stream(side)      # later allocations on the caller's stream come after the side work
+
+    def _tiled_segments(self, nb):
+        """Whether checkpoint segments run on the tiled kernels: FP32 on an even-sized
+        grid, and (for 'auto') a batch of at most half as many propagations as the
+        GPU has CUs, which the one-workgroup-per-propagation kernels would leave
+        mostly idle."""
+        g = self.geom
+        fits = (self.dtype == torch.float32 and self.H * self.W <= 131072 and self.pad <= 44
+                and g.nt * self.H * self.W < 2 ** 29)          # the tiled kernels' limits
+        if self.segment_impl == 'tiled':
+            if not fits:
+                raise ValueError('tiled segments need FP32 and H*W <= 131072, pad <= 44, nt*H*W < 2^29')
+            return True
+        if self.segment_impl == 'resident' or not fits or self.impl != _lib.IMPL_AUTO:
+            return False
+        cus = torch.cuda.get_device_properties(self.device).multi_processor_count
+        return 2 * nb * g.ns <= cus
+
+    def _sweep_group(self, prep, direction, t0, t1, resume, phase, state, hist, res=None, gacc=None, illum=None,
+                     sg=None):
+        grp = (_lib.SSSweepGroup * 1)()
+        g = grp[0]
+        g.geom = self._geom(prep['nb'])
+        g.tile_h, g.tile_w, g.depth = self._seg_tiles.get(direction, (0, 0, 0))
+        g.c2, g.A, g.B, g.w = _lib.ptr(prep['c2']), _lib.ptr(self.A), _lib.ptr(self.B), _lib.ptr(self.w)
+        g.amp = _lib.ptr(prep['amp'] if direction == 0 else prep['sd'])
+        g.state, g.hist = _lib.ptr(state), _lib.ptr(hist)
+        if direction == 1:
+            g.res, g.gacc, g.illum, g.src_grad = _lib.ptr(res), _lib.ptr(gacc), _lib.ptr(illum), _lib.ptr(sg)
+        g.t0, g.t1, g.resume, g.phase = t0, t1, int(resume), phase
+        return grp
+
+    def _segments_tiled(self, prep, tape, res, gacc, illum, sg):
+        """Checkpointed adjoint on the tiled kernels (gpu.py:169-184), for batches too
+        small to fill the GPU one workgroup per propagation: per segment, last
+        first, ss_sweep_forward_f32 recomputes [t0, t1) from the checkpoint
+        (resume) and keeps the Laplacians, then ss_sweep_adjoint_f32 runs [t0, t1)
+        backwards from the previous segment's adjoint state, every propagation
+        split into tiles over all CUs.  Per cell the arithmetic is that of the
+        resident path, so the results are the same bit for bit."""
+        g = self.geom
+        nb = prep['nb']
+        H, W = self.H, self.W
+        ck, seg = tape['ckpt'], tape['seg']
+        nseg = ck.shape[2]
+        plane = nb * g.ns * H * W
+        fst = self._scratch('seg_fstate', (4 * plane,))     # forward: 2 buffers x (p0, p1)
+        ast = self._scratch('seg_astate', (4 * plane,))     # adjoint: 2 buffers x (m1, m2)
+        hist = self._scratch('hist_seg0', (nb, g.ns, seg, H, W))
+        lib = _lib.load()
+        st = self._stream()
+        for d in (0, 1):            # tilings for this batch: tuned once per Sim and direction
+            if d not in self._seg_tiles:
+                grp = self._sweep_group(prep, d, 0, seg, 0, 0, fst if d == 0 else ast, hist, res, gacc, illum, sg)
+                _lib.check(lib.ss_sweep_autotune_f32(1, grp, d, 0, st), 'sweep autotune')
+                self._seg_tiles[d] = (grp[0].tile_h, grp[0].tile_w, grp[0].depth)
+        phase = 0                   # adjoint state buffer that holds (m1, m2)
+        for k in reversed(range(nseg)):
+            t0, t1 = k * seg, min(g.nt, (k + 1) * seg)
+            h = hist.reshape(-1)[:nb * g.ns * (t1 - t0) * H * W]
+            fst[:2 * plane].view(2, nb, g.ns, H, W).copy_(ck[:, :, k].permute(2, 0, 1, 3, 4))
+            end = self._timed('forward')
+            _lib.check(lib.ss_sweep_forward_f32(1, self._sweep_group(prep, 0, t0, t1, 1, 0, fst, h), 1, st),
+                       'sweep forward')
+            if end is not None:
+                end.record()
+            end = self._timed('adjoint')
+            _lib.check(lib.ss_sweep_adjoint_f32(1, self._sweep_group(prep, 1, t0, t1, k < nseg - 1, phase, ast, h,
+                                                                    res, gacc, illum, sg), 1, st), 'sweep adjoint')
+            if end is not None:
+                end.record()
+            phase = (phase - (-(t1 - t0) // self._seg_tiles[1][2])) & 1
 
     def _gradient_device(self, prep, observed=None, obs_repeat=1, v_mute=None, t_mute=0.12, res=None):
         """Forward, muted residual, exact adjoint and fold on the device.

@@ -55,9 +55,9 @@ def test_geometry_struct_layout():
 
 
 def test_sweep_group_struct_layout():
-    # ss_geom (39 int32) + 3 int32 tiling fields, then 12 pointers at 8-byte alignment
-    assert ctypes.sizeof(_lib.SSSweepGroup) == 4 * (7 + _lib.MAX_SHOTS) + 12 + 12 * 8
-    assert _lib.SSSweepGroup.c2.offset == 168
+    # ss_geom (39 int32) + 3 int32 tiling fields, 12 pointers at 8-byte alignment, 4 int32 range fields
+    assert ctypes.sizeof(_lib.SSSweepGroup) == 4 * (7 + _lib.MAX_SHOTS) + 12 + 12 * 8 + 16
+    assert _lib.SSSweepGroup.c2.offset == 168 and _lib.SSSweepGroup.t0.offset == 264
 
 
 def test_sweep_validation_without_gpu_work():
@@ -80,6 +80,12 @@ def test_sweep_validation_without_gpu_work():
     rc = lib.ss_sweep_forward_f32(1, arr, 0, None)
     assert rc == -3 and b'cells per workgroup' in lib.ss_last_error()
     assert lib.ss_sweep_autotune_f32(1, arr, 2, 0, None) == -1
+    arr[0].geom = _lib.make_geom(2, 3, 20, 30, 8, 50, 1, (0, 15, 29))
+    arr[0].tile_h = arr[0].tile_w = arr[0].depth = 0
+    for t0, t1 in ((-1, 10), (10, 10), (20, 51), (50, 0)):
+        arr[0].t0, arr[0].t1 = t0, t1
+        rc = lib.ss_sweep_forward_f32(1, arr, 0, None)
+        assert rc == -1 and b'step range' in lib.ss_last_error(), (t0, t1)
 
 
 def test_workspace_bytes_closed_form():

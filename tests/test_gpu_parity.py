@@ -97,18 +97,39 @@ def test_checkpoint_recompute_equals_full_history(golden):
     case('checkpoint vs full history bitwise')
     obs = np.repeat(golden['obs_f32'], 3, 0)
     full = gpu.Sim(device='cuda', history='full').images(golden['backgrounds'], obs, v_mute=golden['v_mute'])
-    sim = gpu.Sim(device='cuda', history='checkpoint', checkpoint_interval=73)
-    ck = sim.images(golden['backgrounds'], obs, v_mute=golden['v_mute'])
-    np.testing.assert_array_equal(full[0], ck[0])
-    np.testing.assert_array_equal(full[1], ck[1])
-    # the segment recompute overlaps the adjoint on a side stream: a second call on the
-    # same Sim (reused buffers and events) and an interval of 1000 (one segment) agree too
-    again = sim.images(golden['backgrounds'], obs, v_mute=golden['v_mute'])
-    one = gpu.Sim(device='cuda', history='checkpoint', checkpoint_interval=1000).images(
-        golden['backgrounds'], obs, v_mute=golden['v_mute'])
-    for other in (again, one):
-        np.testing.assert_array_equal(full[0], other[0])
-        np.testing.assert_array_equal(full[1], other[1])
+    # resident segments (recompute overlapped on a side stream) and tiled segments (the sweep
+    # kernels over each segment's step range, resuming state): a second call on the same Sim
+    # (reused buffers, events, graphs) and an interval of 1000 (one segment) agree too
+    for seg_impl in ('resident', 'tiled', 'auto'):
+        sim = gpu.Sim(device='cuda', history='checkpoint', checkpoint_interval=73, segment_impl=seg_impl)
+        ck = sim.images(golden['backgrounds'], obs, v_mute=golden['v_mute'])
+        again = sim.images(golden['backgrounds'], obs, v_mute=golden['v_mute'])
+        one = gpu.Sim(device='cuda', history='checkpoint', checkpoint_interval=1000, segment_impl=seg_impl).images(
+            golden['backgrounds'], obs, v_mute=golden['v_mute'])
+        for other in (ck, again, one):
+            np.testing.assert_array_equal(full[0], other[0], err_msg=seg_impl)
+            np.testing.assert_array_equal(full[1], other[1], err_msg=seg_impl)
+
+
+@pytest.mark.parametrize('nx', [28, 27])
+def test_tiled_segments_equal_resident_segments(nx):
+    """Checkpointed gradients on a small grid (paired kernels for an even width, cell
+    kernels for an odd one): tiled segments, resident segments and the whole history
+    agree bit for bit, with a short last segment (nt 260, interval 40)."""
+    case(f'tiled vs resident checkpoint segments 24x{nx} nt=260 ns=3 interval 40')
+    geom = Acquisition(nz=24, nx=nx, nt=260, ns=3)
+    rng = np.random.default_rng(5)
+    v = 1800 + 900 * rng.random((2, 24, nx))
+    obs = gpu.Sim(device='cuda', pad=12, geom=geom).gathers(v) * 0.9
+    vm = np.array([1900.0, 2100.0])
+    want = gpu.Sim(device='cuda', pad=12, geom=geom, history='full').gradient(v, obs, v_mute=vm)
+    for seg_impl in ('tiled', 'resident'):
+        sim = gpu.Sim(device='cuda', pad=12, geom=geom, history='checkpoint', checkpoint_interval=40,
+                      segment_impl=seg_impl)
+        for _ in range(2):
+            got = sim.gradient(v, obs, v_mute=vm)
+            np.testing.assert_array_equal(got[0], want[0], err_msg=seg_impl)
+            np.testing.assert_array_equal(got[1], want[1], err_msg=seg_impl)
 
 
 @pytest.mark.parametrize('dtype', [torch.float32, torch.float64])
