@@ -1031,29 +1031,48 @@ int run_plan(const Plan &p, int use_graph, hipStream_t st) {
   return hip_status(hipGraphLaunch(exec, st), "sweep graph launch");
 }
 
-// Time the first (up to) three launches of one group alone under a tiling,
-// after one untimed launch; returns ms per cell-step, negative on error.
+// Kernel variant and LDS floors a group's items get in a joint launch: the
+// joint plan runs every item with the largest slot count and LDS size of all
+// its groups.
+struct Floors {
+  bool set = false, paired = false;
+  int K = 0;
+  size_t lds = 0;
+};
+
+// Time the first (up to) six launches of one group alone under a tiling, the
+// faster of two runs after one untimed launch, with the kernel variant and LDS
+// raised to the floors (as in the joint launch); ms per cell-step, negative on
+// error.
 double time_choice(const ss_sweep_group &grp, int dir, const TileChoice &tc, hipStream_t st, hipEvent_t e0,
-                   hipEvent_t e1) {
+                   hipEvent_t e1, const Floors &fl) {
   ss_sweep_group g1 = grp;
   g1.tile_h = tc.th;
   g1.tile_w = tc.tw;
   g1.depth = tc.T;
   Plan p;
   if (make_plan(1, &g1, dir, &p) != SS_OK) return -1.0;
+  if (fl.set && fl.paired == p.paired) {
+    p.K = std::max(p.K, fl.K);
+    p.lds = std::max(p.lds, fl.lds);
+  }
   KernelFn fn;
   if (prepare_kernel(p, &fn) != SS_OK) return -1.0;
   const SGroup &G = p.args.g[0];
-  const int m = std::min(p.nlaunch, 3);
+  const int m = std::min(p.nlaunch, 6);
   launch_range(p, fn, 0, 1, st);
-  if (hipEventRecord(e0, st) != hipSuccess) return -1.0;
-  launch_range(p, fn, 0, m, st);
-  if (hipEventRecord(e1, st) != hipSuccess) return -1.0;
-  if (hipEventSynchronize(e1) != hipSuccess) return -1.0;
-  float ms = 0.0f;
-  if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess) return -1.0;
+  float best = -1.0f;
+  for (int rep = 0; rep < 2; ++rep) {  // the faster of two timings of the first m launches
+    if (hipEventRecord(e0, st) != hipSuccess) return -1.0;
+    launch_range(p, fn, 0, m, st);
+    if (hipEventRecord(e1, st) != hipSuccess) return -1.0;
+    if (hipEventSynchronize(e1) != hipSuccess) return -1.0;
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess) return -1.0;
+    if (best < 0.0f || ms < best) best = ms;
+  }
   const double steps = (double)std::min(G.thi - G.tlo, m * G.T);
-  return ms / (steps * (double)G.nitems / G.ntiles * G.H * G.W);
+  return best / (steps * (double)G.nitems / G.ntiles * G.H * G.W);
 }
 
 }  // namespace
@@ -1079,7 +1098,20 @@ int sweep_autotune(int ngroups, ss_sweep_group *groups, int dir, int max_trials,
     return hip_status(e, "autotune event");
   }
   int rc = SS_OK;
-  for (int i = 0; i < ngroups && rc == SS_OK; ++i) {
+  // Largest groups first (cell-steps): in the joint launch their tilings set the kernel
+  // variant and LDS size that every smaller group's items run with, so those are timed
+  // under the same floors.
+  std::vector<int> order(ngroups);
+  std::vector<double> work(ngroups, 0.0);
+  for (int i = 0; i < ngroups; ++i) {
+    order[i] = i;
+    const ss_geom &q = groups[i].geom;
+    work[i] = (double)q.nb * q.ns * q.nt * (q.nz + 2.0 * q.pad) * (q.nx + 2.0 * q.pad);
+  }
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return work[a] > work[b]; });
+  Floors fl;
+  for (int oi = 0; oi < ngroups && rc == SS_OK; ++oi) {
+    const int i = order[oi];
     Geom g;
     rc = check_geom(&groups[i].geom, &g);
     if (rc != SS_OK) break;
@@ -1088,7 +1120,7 @@ int sweep_autotune(int ngroups, ss_sweep_group *groups, int dir, int max_trials,
     TileChoice pick{0, 0, 0};
     const int ntry = std::min<int>((int)cand.size(), max_trials > 0 ? max_trials : 12);
     for (int c = 0; c < ntry; ++c) {
-      const double tpc = time_choice(groups[i], dir, cand[c], st, e0, e1);
+      const double tpc = time_choice(groups[i], dir, cand[c], st, e0, e1, fl);
       if (tpc > 0.0 && (best < 0.0 || tpc < best)) {
         best = tpc;
         pick = cand[c];
@@ -1102,6 +1134,17 @@ int sweep_autotune(int ngroups, ss_sweep_group *groups, int dir, int max_trials,
     groups[i].tile_h = pick.th;
     groups[i].tile_w = pick.tw;
     groups[i].depth = pick.T;
+    Plan pp;
+    if (make_plan(1, &groups[i], dir, &pp) == SS_OK) {
+      if (!fl.set) {
+        fl.set = true;
+        fl.paired = pp.paired;
+      }
+      if (fl.paired == pp.paired) {
+        fl.K = std::max(fl.K, pp.K);
+        fl.lds = std::max(fl.lds, pp.lds);
+      }
+    }
     std::lock_guard<std::mutex> lock(g_mutex);
     g_tuned[tune_key(dir, g.H, g.W, g.pad)] = pick;
   }

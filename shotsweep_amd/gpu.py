@@ -480,21 +480,28 @@ class Sim:
         hist = self._scratch('hist_seg0', (nb, g.ns, seg, H, W))
         lib = _lib.load()
         st = self._stream()
-        for d in (0, 1):            # tilings for this batch: tuned once per Sim and direction
+
+        def tune(d, t0, t1):        # tilings for this batch, tuned once per Sim and direction
             if d not in self._seg_tiles:
-                grp = self._sweep_group(prep, d, 0, seg, 0, 0, fst if d == 0 else ast, hist, res, gacc, illum, sg)
+                grp = self._sweep_group(prep, d, t0, t1, 0, 0, fst if d == 0 else ast, hist, res, gacc, illum, sg)
                 _lib.check(lib.ss_sweep_autotune_f32(1, grp, d, 0, st), 'sweep autotune')
                 self._seg_tiles[d] = (grp[0].tile_h, grp[0].tile_w, grp[0].depth)
+
         phase = 0                   # adjoint state buffer that holds (m1, m2)
         for k in reversed(range(nseg)):
             t0, t1 = k * seg, min(g.nt, (k + 1) * seg)
             h = hist.reshape(-1)[:nb * g.ns * (t1 - t0) * H * W]
+            tune(0, t0, t1)          # forward trials overwrite the segment buffers, so tune first
             fst[:2 * plane].view(2, nb, g.ns, H, W).copy_(ck[:, :, k].permute(2, 0, 1, 3, 4))
             end = self._timed('forward')
             _lib.check(lib.ss_sweep_forward_f32(1, self._sweep_group(prep, 0, t0, t1, 1, 0, fst, h), 1, st),
                        'sweep forward')
             if end is not None:
                 end.record()
+            if 1 not in self._seg_tiles:
+                # adjoint trials read this segment's history (valid now); the first real adjoint
+                # call (resume 0) then overwrites the outputs the trials left
+                tune(1, t0, t1)
             end = self._timed('adjoint')
             _lib.check(lib.ss_sweep_adjoint_f32(1, self._sweep_group(prep, 1, t0, t1, k < nseg - 1, phase, ast, h,
                                                                     res, gacc, illum, sg), 1, st), 'sweep adjoint')
