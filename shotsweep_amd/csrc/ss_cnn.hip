@@ -34,6 +34,7 @@ __global__ __launch_bounds__(kTile *kTile) void conv3x3_kernel(int N, int Cin, i
   const int ty0 = (blockIdx.x / tiles_x) * kTile, tx0 = (blockIdx.x % tiles_x) * kTile;
   const int co0 = blockIdx.y * kCoGroup;
   const int n = blockIdx.z;
+  if (n >= N) return;  // whole workgroup, before any barrier
   const int tid = threadIdx.x;
   for (int k = tid; k < Cin * (kTile + 2) * (kTile + 2); k += kTile * kTile) {
     const int ci = k / ((kTile + 2) * (kTile + 2));

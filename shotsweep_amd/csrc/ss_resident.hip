@@ -68,9 +68,6 @@ constexpr uint32_t kModel = 1u << 14;
 // above it: otherwise the compiler interleaves all cells of a step and runs
 // out of registers for the per-cell state.
 constexpr int kGroup = SS_GROUP;
-__device__ __forceinline__ void group_fence(int j, int &off) {
-  if (j % kGroup == 0) asm volatile("" : "+v"(off));
-}
 __device__ __forceinline__ void group_fence(int j, int &off, uint32_t *pk) {
   if (j % kGroup == 0) {
     asm volatile("" : "+v"(off));
