@@ -19,7 +19,6 @@ the autotuner (``ss_sweep_autotune_f32``) on the first call.  When the
 Laplacian histories of all groups exceed the memory budget, the models are
 split into chunks, each a proportional slice of every group.
 """
-import ctypes
 import math
 
 import numpy as np
