@@ -84,11 +84,14 @@ __host__ __device__ __forceinline__ T lapt_sum(T qc, T qzp2, T qzp1, T qzm1, T q
 // Exact zeros, common ahead of the wavefront, stay on the fast path.
 constexpr int kZeroOrNormal = 0x168;  // __builtin_isfpclass: -normal, -zero, +zero, +normal
 
+// The remainder is formed as -(12 q0 - s) rather than s - 12 q0: the same value
+// for every s except zeros, where the negated form carries the sign that makes
+// the final FMA return s's signed zero (no separate copysign).
 __host__ __device__ __forceinline__ float div12_fast(float s) {
   const float R = 0x1.555556p-4f;  // RN(1/12)
   const float q0 = s * R;
-  const float r = __builtin_fmaf(-q0, 12.0f, s);
-  return __builtin_copysignf(__builtin_fmaf(r, R, q0), s);
+  const float r = -__builtin_fmaf(q0, 12.0f, -s);
+  return __builtin_fmaf(r, R, q0);
 }
 
 __host__ __device__ __forceinline__ float div12(float s) {

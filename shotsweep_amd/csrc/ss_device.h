@@ -60,6 +60,8 @@ __device__ __forceinline__ float2 lds_table2(uint32_t byte_off) {
   const ss_v2f v = *reinterpret_cast<ss_lds_cv2f *>((uintptr_t)byte_off);
   return make_float2(v.x, v.y);
 }
+// 8-byte read at an LDS byte address (same addressing as the tables).
+__device__ __forceinline__ float2 lds_ld2(uint32_t byte_addr) { return lds_table2(byte_addr); }
 // (A, 2A, A*B) of a 16-byte adjoint entry
 __device__ __forceinline__ float3 lds_table3(uint32_t byte_off) {
   const ss_v2f v = *reinterpret_cast<ss_lds_cv2f *>((uintptr_t)byte_off);

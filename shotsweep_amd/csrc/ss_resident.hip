@@ -335,6 +335,7 @@ __global__ __launch_bounds__(kFwdThreads) void fwd_pair_kernel(Geom g, int t0, i
   float2 *tab = reinterpret_cast<float2 *>(lds);  // at LDS address 0: lookups need no base add
   float *P = lds + tabf + guard;
   float2 *P2 = reinterpret_cast<float2 *>(P);
+  constexpr uint32_t kPAddr = (uint32_t)(tabf + guard) * 4;  // LDS byte address of P
   const int64_t prop = xcd_prop(g);
   if (prop < 0) return;  // padding workgroup (whole workgroup exits before any barrier)
   const int b = (int)(prop / g.ns), s = (int)(prop - (int64_t)b * g.ns);
@@ -348,6 +349,9 @@ __global__ __launch_bounds__(kFwdThreads) void fwd_pair_kernel(Geom g, int t0, i
 
   float2 pn[KP], cc[KP];
   uint32_t pk[KP];  // two 16-bit cell descriptors per pair
+  // Laplacian-interior flag of a pair: the 2-cell rim has even width, so both
+  // cells of a pair share it.  Kept as a lane mask, it costs one select per cell.
+  bool inner[KP];
   const float2 *p0p = reinterpret_cast<const float2 *>(p0g + prop * HW);
   const float2 *p1p = reinterpret_cast<const float2 *>(p1g + prop * HW);
   const float2 *c2b = reinterpret_cast<const float2 *>(c2 + (int64_t)b * HW);
@@ -356,11 +360,13 @@ __global__ __launch_bounds__(kFwdThreads) void fwd_pair_kernel(Geom g, int t0, i
     const int q = t + j * kFwdThreads;
     pn[j] = cc[j] = make_float2(0.0f, 0.0f);
     pk[j] = 0;
+    inner[j] = false;
     if (q < NP) {
       pn[j] = p0p[q];
       P2[q] = p1p[q];
       cc[j] = c2b[q];
       pk[j] = pair_cell_desc<3>(2 * q, H, W, nz, nx, pad) | (pair_cell_desc<3>(2 * q + 1, H, W, nz, nx, pad) << 16);
+      inner[j] = (pk[j] & kPInterior) != 0;
     }
   }
   constexpr int rec0 = (pad + row) * W + pad;
@@ -370,7 +376,10 @@ __global__ __launch_bounds__(kFwdThreads) void fwd_pair_kernel(Geom g, int t0, i
   const int nslots = t1 - t0;
   const int nck = ckpt ? (nslots + ckpt_every - 1) / ckpt_every : 0;
   const bool rec_thread = gathers && t < nx;
-  const __amdgpu_buffer_rsrc_t hrs = make_rsrc(hist ? hist + prop * nslots * HW : p0g, (int64_t)nslots * HW);
+  // Without a history buffer the resource has size 0 and the hardware drops
+  // the Laplacian stores, so the step loop needs no branch for them.
+  const __amdgpu_buffer_rsrc_t hrs =
+      make_rsrc(hist ? hist + prop * nslots * HW : p0g, hist ? (int64_t)nslots * HW : 0);
   const __amdgpu_buffer_rsrc_t grs =
       make_rsrc(gathers ? gathers + prop * g.nt * nx : p0g, (int64_t)g.nt * nx);
   const __amdgpu_buffer_rsrc_t crs = make_rsrc(ckpt ? ckpt + prop * nck * 2 * HW : p0g, nck * 2 * HW);
@@ -397,18 +406,22 @@ __global__ __launch_bounds__(kFwdThreads) void fwd_pair_kernel(Geom g, int t0, i
     for (int j = 0; j < KP; ++j) {
       asm volatile("" : "+v"(pk[j]));  // one pair at a time
       if (t + j * kFwdThreads < NP) {
-        const int q = off + j * kFwdThreads;
         const uint32_t d0 = pk[j] & 0xffffu, d1 = pk[j] >> 16;
-        const float2 r0 = P2[q - 1], r1 = P2[q], r2 = P2[q + 1];
-        const float2 um = P2[q - W2], u2m = P2[q - W], up = P2[q + W2], u2p = P2[q + W];
+        // One base address per pair (two rows up); the seven taps are
+        // immediate offsets from it.
+        uint32_t a = kPAddr + 8u * (uint32_t)(off + j * kFwdThreads - W);
+        asm volatile("" : "+v"(a));
+        const float2 u2m = lds_ld2(a), um = lds_ld2(a + 4 * W), r0 = lds_ld2(a + 8 * W - 8);
+        const float2 r1 = lds_ld2(a + 8 * W), r2 = lds_ld2(a + 8 * W + 8), up = lds_ld2(a + 12 * W);
+        const float2 u2p = lds_ld2(a + 16 * W);
         const float s0 = lap_sum(u2m.x, um.x, r1.x, up.x, u2p.x, r0.x, r0.y, r1.y, r2.x);
         const float s1 = lap_sum(u2m.y, um.y, r1.y, up.y, u2p.y, r0.y, r1.x, r2.x, r2.y);
         const float2 Lq = div12x2(s0, s1);
-        const float L0 = keep_if<0>(Lq.x, pk[j]), L1 = keep_if<16>(Lq.y, pk[j]);  // interior flags
+        const float L0 = inner[j] ? Lq.x : 0.0f, L1 = inner[j] ? Lq.y : 0.0f;
         const float2 ab0 = sponge_at(d0), ab1 = sponge_at(d1);
         pn[j].x = leapfrog(r1.x, pn[j].x, ab0.y, cc[j].x, L0, ab0.x);
         pn[j].y = leapfrog(r1.y, pn[j].y, ab1.y, cc[j].y, L1, ab1.x);
-        if (hist) buf_store2(make_float2(L0, L1), hrs, off * 8, (slot * HW + 2 * j * kFwdThreads) * 4);
+        buf_store2(make_float2(L0, L1), hrs, off * 8, (slot * HW + 2 * j * kFwdThreads) * 4);
         asm volatile("" : "+v"(pn[j].x), "+v"(pn[j].y));  // finish this pair before the next starts
       }
     }
@@ -625,6 +638,7 @@ __global__ __launch_bounds__(kAdjThreads) void adj_pair_kernel(Geom g, int t0, i
   float4 *tab = reinterpret_cast<float4 *>(lds);  // at LDS address 0: lookups need no base add
   float *Q = lds + tabf + guard;
   float2 *Q2 = reinterpret_cast<float2 *>(Q);
+  constexpr uint32_t kQAddr = (uint32_t)(tabf + guard) * 4;  // LDS byte address of Q
   float *il = Q + HW + guard;
   float2 *il2 = reinterpret_cast<float2 *>(il);
   const int64_t prop = xcd_prop(g);
@@ -643,7 +657,10 @@ __global__ __launch_bounds__(kAdjThreads) void adj_pair_kernel(Geom g, int t0, i
 #pragma unroll
   for (int j = 0; j < KP; ++j) {
     const int q = t + j * kAdjThreads;
-    pk[j] = q < NP ? (pair_cell_desc<4>(2 * q, H, W, nz, nx, pad) | (pair_cell_desc<4>(2 * q + 1, H, W, nz, nx, pad) << 16))
+    // bit 31 (the second cell's unused bit 15) repeats the model flag, so one
+    // signed compare tests it
+    const uint32_t d0 = pair_cell_desc<4>(2 * q, H, W, nz, nx, pad);
+    pk[j] = q < NP ? (d0 | (pair_cell_desc<4>(2 * q + 1, H, W, nz, nx, pad) << 16) | ((d0 & kPModel) ? 0x80000000u : 0u))
                    : 0u;
   }
   const __amdgpu_buffer_rsrc_t srs1 = make_rsrc(m1g + prop * HW, HW);
@@ -692,26 +709,31 @@ __global__ __launch_bounds__(kAdjThreads) void adj_pair_kernel(Geom g, int t0, i
   for (int it = t1 - 1; it >= t0; --it) {
     const int slot = it - t0;
     const float wt = w[it];
-    int off = t;
+    uint32_t o8 = 8u * t;  // the thread's byte offset within a row of pairs
     opaque(pk);
 #pragma unroll
     for (int j = 0; j < KP; ++j) {
-      asm volatile("" : "+v"(off), "+v"(pk[j]));  // one pair at a time
+      asm volatile("" : "+v"(o8), "+v"(pk[j]));  // one pair at a time
       const float2 L = Lb[j % D];
       {
         const int jn = j + D;
         if (jn < KP) {
           if (t + jn * kAdjThreads < NP)
-            Lb[j % D] = buf_load2(hrs, off * 8, (slot * HW + 2 * jn * kAdjThreads) * 4);
+            Lb[j % D] = buf_load2(hrs, o8, (slot * HW + 2 * jn * kAdjThreads) * 4);
         } else if (it > t0 && t + (jn - KP) * kAdjThreads < NP) {
-          Lb[j % D] = buf_load2(hrs, off * 8, ((slot - 1) * HW + 2 * (jn - KP) * kAdjThreads) * 4);
+          Lb[j % D] = buf_load2(hrs, o8, ((slot - 1) * HW + 2 * (jn - KP) * kAdjThreads) * 4);
         }
       }
       if (t + j * kAdjThreads < NP) {
-        const int q = off + j * kAdjThreads;
+        const int q = (int)(o8 >> 3) + j * kAdjThreads;
         const uint32_t d0 = pk[j] & 0xffffu, d1 = pk[j] >> 16;
-        const float2 r0 = Q2[q - 1], r1 = Q2[q], r2 = Q2[q + 1];
-        const float2 um = Q2[q - W2], u2m = Q2[q - W], up = Q2[q + W2], u2p = Q2[q + W];
+        // One base address per pair (two rows up); the seven taps are
+        // immediate offsets from it.
+        uint32_t a = o8 + (kQAddr + 8u * (uint32_t)(j * kAdjThreads - W));
+        asm volatile("" : "+v"(a));
+        const float2 u2m = lds_ld2(a), um = lds_ld2(a + 4 * W), r0 = lds_ld2(a + 8 * W - 8);
+        const float2 r1 = lds_ld2(a + 8 * W), r2 = lds_ld2(a + 8 * W + 8), up = lds_ld2(a + 12 * W);
+        const float2 u2p = lds_ld2(a + 16 * W);
         const float lt0 = lapt_sum(r1.x, u2p.x, up.x, um.x, u2m.x, r2.x, r1.y, r0.y, r0.x);
         const float lt1 = lapt_sum(r1.y, u2p.y, up.y, um.y, u2m.y, r2.y, r2.x, r1.x, r0.y);
         const float3 aab0 = sponge4_at(d0), aab1 = sponge4_at(d1);  // (A, 2A, A B)
@@ -733,7 +755,7 @@ __global__ __launch_bounds__(kAdjThreads) void adj_pair_kernel(Geom g, int t0, i
         if (j == srcq / kAdjThreads) {
           if (q == srcq) sg = sg + (((srci & 1) ? mu1 : mu0) * wt) * sdv;
         }
-        if (d0 & kPModel) {
+        if ((int32_t)pk[j] < 0) {  // model pair
           const int c = 2 * q, z = c / W, x = c - z * W;
           float2 &v = il2[((z - pad) * nx + (x - pad)) >> 1];
           v.x = v.x + L.x * L.x;
@@ -746,7 +768,7 @@ __global__ __launch_bounds__(kAdjThreads) void adj_pair_kernel(Geom g, int t0, i
       }
     }
     __syncthreads();
-    off = t;
+    int off = t;
     asm volatile("" : "+v"(off));  // keep per-cell descriptors from being hoisted out of the time loop
     opaque(pk);
 #pragma unroll
@@ -761,7 +783,8 @@ __global__ __launch_bounds__(kAdjThreads) void adj_pair_kernel(Geom g, int t0, i
       if (t + j * kAdjThreads < NP) {
         const uint32_t d0 = pk[j] & 0xffffu, d1 = pk[j] >> 16;
         const float2 qq = div12x2((cv.x * sponge4_a(d0)) * m1[j].x, (cv.y * sponge4_a(d1)) * m1[j].y);
-        Q2[off + j * kAdjThreads] = make_float2(keep_if<0>(qq.x, pk[j]), keep_if<16>(qq.y, pk[j]));
+        // both cells of a pair share the interior flag (bit 0)
+        Q2[off + j * kAdjThreads] = make_float2(keep_if<0>(qq.x, pk[j]), keep_if<0>(qq.y, pk[j]));
       }
     }
     __syncthreads();
