@@ -968,9 +968,19 @@ void launch_range(const Plan &p, KernelFn fn, int k0, int k1, hipStream_t st) {
 
 // ------------------------------------------------------------------- graphs
 
+// Executable graphs by batch layout, least recently used evicted beyond
+// kMaxGraphs.  A checkpointed gradient replays two graphs per segment (forward
+// recompute and adjoint, each with its own step range), so the cache holds the
+// 40 layouts of 1000 steps in segments of 50 several times over.
+constexpr size_t kMaxGraphs = 256;
+struct CachedGraph {
+  hipGraphExec_t exec;
+  uint64_t last_use;
+};
 struct GraphCache {
-  std::map<std::string, hipGraphExec_t> exec;
+  std::map<std::string, CachedGraph> exec;
   std::map<int, hipStream_t> capture_stream;
+  uint64_t clock = 0;
 };
 GraphCache g_graphs;
 
@@ -997,7 +1007,8 @@ int run_plan(const Plan &p, int use_graph, hipStream_t st) {
   auto it = g_graphs.exec.find(key);
   hipGraphExec_t exec = nullptr;
   if (it != g_graphs.exec.end()) {
-    exec = it->second;
+    exec = it->second.exec;
+    it->second.last_use = ++g_graphs.clock;
   } else {
     hipStream_t cs;
     auto sit = g_graphs.capture_stream.find(dev);
@@ -1008,9 +1019,15 @@ int run_plan(const Plan &p, int use_graph, hipStream_t st) {
     } else {
       cs = sit->second;
     }
-    if (g_graphs.exec.size() >= 32) {
-      for (auto &kv : g_graphs.exec) (void)hipGraphExecDestroy(kv.second);
-      g_graphs.exec.clear();
+    if (g_graphs.exec.size() >= kMaxGraphs) {
+      auto old = g_graphs.exec.begin();
+      for (auto i = g_graphs.exec.begin(); i != g_graphs.exec.end(); ++i)
+        if (i->second.last_use < old->second.last_use) old = i;
+      // the evicted graph may still be queued on some stream: let it finish first
+      e = hipDeviceSynchronize();
+      if (e != hipSuccess) return hip_status(e, "sweep graph eviction");
+      (void)hipGraphExecDestroy(old->second.exec);
+      g_graphs.exec.erase(old);
     }
     hipGraph_t graph = nullptr;
     e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
@@ -1026,7 +1043,7 @@ int run_plan(const Plan &p, int use_graph, hipStream_t st) {
     e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
     if (e != hipSuccess) return hip_status(e, "sweep graph instantiate");
-    g_graphs.exec[key] = exec;
+    g_graphs.exec[key] = CachedGraph{exec, ++g_graphs.clock};
   }
   return hip_status(hipGraphLaunch(exec, st), "sweep graph launch");
 }
