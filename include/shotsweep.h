@@ -283,7 +283,8 @@ int ss_velocity_scores_f64(int B, int H, int W, const double *pred, const double
  *   res               adjoint: muted residual (nb, ns, nt, nx)
  *   state             scratch, 4 * nb * ns * H * W floats
  *   gathers           forward output (nb, ns, nt, nx) or NULL
- *   hist              Laplacian history (nb, ns, nt, H, W): forward writes, adjoint reads
+ *   hist              Laplacian history (nb, ns, nt, H, W): forward writes, adjoint reads;
+ *                     NULL for a forward that keeps no history (state and gathers only)
  *   gacc, illum, src_grad   adjoint outputs (nb,ns,H,W), (nb,ns,nz,nx), (nb,ns), overwritten
  *   t0, t1            step range of this call, [t0, t1) (t1 = 0: [0, nt)); hist then holds
  *                     (nb, ns, t1-t0, H, W) and gathers / res stay (nb, ns, nt, nx).  The

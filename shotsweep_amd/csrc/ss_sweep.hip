@@ -200,7 +200,10 @@ __global__ __launch_bounds__(kThreads) void sweep_fwd_kernel(SArgs a) {
     }
   }
   const int64_t hspan = G.thi - G.tlo;  // history slots of this call
-  const __amdgpu_buffer_rsrc_t hrs = make_rsrc(G.hist + (int64_t)v.prop * hspan * HW, hspan * HW);
+  // without a history buffer (a forward that only advances the state) the
+  // resource has size 0 and the hardware drops the Laplacian stores
+  const __amdgpu_buffer_rsrc_t hrs =
+      make_rsrc(G.hist ? G.hist + (int64_t)v.prop * hspan * HW : G.st, G.hist ? hspan * HW : 0);
   // own receivers: row r, model columns [xa, xb) of this tile
   const int xa = max(v.x0, pad), xb = min(v.x1, pad + G.nx);
   const bool rec = G.gathers != nullptr && r >= v.y0 && r < v.y1 && t < xb - xa;
@@ -447,7 +450,10 @@ __global__ __launch_bounds__(kThreads) void sweep_fwd_pair_kernel(SArgs a) {
     }
   }
   const int64_t hspan = G.thi - G.tlo;  // history slots of this call
-  const __amdgpu_buffer_rsrc_t hrs = make_rsrc(G.hist + (int64_t)v.prop * hspan * HW, hspan * HW);
+  // without a history buffer (a forward that only advances the state) the
+  // resource has size 0 and the hardware drops the Laplacian stores
+  const __amdgpu_buffer_rsrc_t hrs =
+      make_rsrc(G.hist ? G.hist + (int64_t)v.prop * hspan * HW : G.st, G.hist ? hspan * HW : 0);
   const int xa = max(v.x0, pad), xb = min(v.x1, pad + G.nx);
   const bool rec = G.gathers != nullptr && r >= v.y0 && r < v.y1 && t < xb - xa;
   const int rec_e = (r - v.ey0) * EW + (xa + t - v.ex0);
@@ -766,10 +772,10 @@ int fill_group(const ss_sweep_group &in, int dir, SGroup *o, char *err, size_t e
     snprintf(err, errn, "sweep: step range [%d, %d) outside [0, %d)", in.t0, in.t1, g.nt);
     return SS_EINVAL;
   }
-  const void *need[] = {in.c2, in.A, in.B, in.w, in.amp, in.state, in.hist};
+  const void *need[] = {in.c2, in.A, in.B, in.w, in.amp, in.state, dir ? in.hist : in.state};
   for (const void *p : need)
     if (!p) {
-      snprintf(err, errn, "sweep: c2, A, B, w, amp/sd, state and hist are required");
+      snprintf(err, errn, "sweep: c2, A, B, w, amp/sd, state (and for the adjoint hist) are required");
       return SS_EINVAL;
     }
   if (dir && (!in.res || !in.gacc || !in.illum || !in.src_grad)) {

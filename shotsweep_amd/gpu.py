@@ -346,7 +346,10 @@ class Sim:
             return sim, {'seg': g.nt, 'hist': hist}
         nck = -(-g.nt // seg)
         ck = new('ckpt', (nb, g.ns, nck, 2, H, W))
-        self._forward(geom, 0, g.nt, c2, amp, p0, p1, gathers=sim, ckpt=ck, every=seg)
+        if self._tiled_segments(nb):
+            self._forward_tape_tiled(prep, seg, sim, ck)
+        else:
+            self._forward(geom, 0, g.nt, c2, amp, p0, p1, gathers=sim, ckpt=ck, every=seg)
         return sim, {'seg': seg, 'ckpt': ck}
 
     def _adjoint_tape(self, prep, tape, res):
@@ -448,7 +451,7 @@ class Sim:
         return 2 * nb * g.ns <= cus
 
     def _sweep_group(self, prep, direction, t0, t1, resume, phase, state, hist, res=None, gacc=None, illum=None,
-                     sg=None):
+                     sg=None, gathers=None):
         grp = (_lib.SSSweepGroup * 1)()
         g = grp[0]
         g.geom = self._geom(prep['nb'])
@@ -458,8 +461,49 @@ class Sim:
         g.state, g.hist = _lib.ptr(state), _lib.ptr(hist)
         if direction == 1:
             g.res, g.gacc, g.illum, g.src_grad = _lib.ptr(res), _lib.ptr(gacc), _lib.ptr(illum), _lib.ptr(sg)
+        else:
+            g.gathers = _lib.ptr(gathers)
         g.t0, g.t1, g.resume, g.phase = t0, t1, int(resume), phase
         return grp
+
+    def _seg_tune(self, prep, direction, t0, t1, state, hist, res=None, gacc=None, illum=None, sg=None):
+        """Tile shape and depth of the segment sweeps, tuned once per Sim and direction.
+        The trial launches overwrite the state buffers and the adjoint outputs."""
+        if direction not in self._seg_tiles:
+            grp = self._sweep_group(prep, direction, t0, t1, 0, 0, state, hist, res, gacc, illum, sg)
+            _lib.check(_lib.load().ss_sweep_autotune_f32(1, grp, direction, 0, self._stream()), 'sweep autotune')
+            self._seg_tiles[direction] = (grp[0].tile_h, grp[0].tile_w, grp[0].depth)
+
+    def _forward_tape_tiled(self, prep, seg, sim, ck):
+        """Checkpointed forward (gpu.py:126-136 with checkpoints) on the tiled kernels,
+        for batches too small to fill the GPU one workgroup per propagation: per
+        segment, ss_sweep_forward_f32 advances [t0, t1) from the previous segment's
+        state without keeping Laplacians and records the gathers; the state at each
+        segment start is copied out as that segment's checkpoint.  The gathers and
+        checkpoints equal the resident forward's bit for bit."""
+        g = self.geom
+        nb = prep['nb']
+        H, W = self.H, self.W
+        plane = nb * g.ns * H * W
+        fst = self._scratch('seg_fstate', (4 * plane,))
+        states = fst.view(2, 2, nb, g.ns, H, W)        # buffer, (p0, p1), rows, shots
+        lib = _lib.load()
+        st = self._stream()
+        # trials overwrite the state buffers: tune before the first segment
+        self._seg_tune(prep, 0, 0, min(seg, g.nt), fst, self._scratch('hist_seg0', (nb, g.ns, seg, H, W)))
+        depth = self._seg_tiles[0][2]
+        ck[:, :, 0].zero_()
+        phase = 0
+        for k in range(ck.shape[2]):
+            t0, t1 = k * seg, min(g.nt, (k + 1) * seg)
+            if k > 0:
+                ck[:, :, k].copy_(states[phase].permute(1, 2, 0, 3, 4))
+            end = self._timed('forward')
+            _lib.check(lib.ss_sweep_forward_f32(1, self._sweep_group(prep, 0, t0, t1, k > 0, phase, fst, None,
+                                                                     gathers=sim), 1, st), 'sweep forward')
+            if end is not None:
+                end.record()
+            phase = (phase - (-(t1 - t0) // depth)) & 1
 
     def _segments_tiled(self, prep, tape, res, gacc, illum, sg):
         """Checkpointed adjoint on the tiled kernels (gpu.py:169-184), for batches too
@@ -480,18 +524,11 @@ class Sim:
         hist = self._scratch('hist_seg0', (nb, g.ns, seg, H, W))
         lib = _lib.load()
         st = self._stream()
-
-        def tune(d, t0, t1):        # tilings for this batch, tuned once per Sim and direction
-            if d not in self._seg_tiles:
-                grp = self._sweep_group(prep, d, t0, t1, 0, 0, fst if d == 0 else ast, hist, res, gacc, illum, sg)
-                _lib.check(lib.ss_sweep_autotune_f32(1, grp, d, 0, st), 'sweep autotune')
-                self._seg_tiles[d] = (grp[0].tile_h, grp[0].tile_w, grp[0].depth)
-
         phase = 0                   # adjoint state buffer that holds (m1, m2)
         for k in reversed(range(nseg)):
             t0, t1 = k * seg, min(g.nt, (k + 1) * seg)
             h = hist.reshape(-1)[:nb * g.ns * (t1 - t0) * H * W]
-            tune(0, t0, t1)          # forward trials overwrite the segment buffers, so tune first
+            self._seg_tune(prep, 0, t0, t1, fst, hist)   # forward trials overwrite the segment buffers: tune first
             fst[:2 * plane].view(2, nb, g.ns, H, W).copy_(ck[:, :, k].permute(2, 0, 1, 3, 4))
             end = self._timed('forward')
             _lib.check(lib.ss_sweep_forward_f32(1, self._sweep_group(prep, 0, t0, t1, 1, 0, fst, h), 1, st),
@@ -501,7 +538,7 @@ class Sim:
             if 1 not in self._seg_tiles:
                 # adjoint trials read this segment's history (valid now); the first real adjoint
                 # call (resume 0) then overwrites the outputs the trials left
-                tune(1, t0, t1)
+                self._seg_tune(prep, 1, t0, t1, ast, hist, res, gacc, illum, sg)
             end = self._timed('adjoint')
             _lib.check(lib.ss_sweep_adjoint_f32(1, self._sweep_group(prep, 1, t0, t1, k < nseg - 1, phase, ast, h,
                                                                     res, gacc, illum, sg), 1, st), 'sweep adjoint')
