@@ -79,6 +79,13 @@ def test_sweep_validation_without_gpu_work():
     arr[0].tile_h, arr[0].tile_w, arr[0].depth = 200, 200, 40
     rc = lib.ss_sweep_forward_f32(1, arr, 0, None)
     assert rc == -3 and b'cells per workgroup' in lib.ss_last_error()
+    # a forward may keep no Laplacian history (hist NULL); the adjoint needs one
+    arr[0].hist = None
+    rc = lib.ss_sweep_forward_f32(1, arr, 0, None)
+    assert rc == -3 and b'cells per workgroup' in lib.ss_last_error()   # past the pointer checks
+    rc = lib.ss_sweep_adjoint_f32(1, arr, 0, None)
+    assert rc == -1 and b'hist' in lib.ss_last_error()
+    arr[0].hist = 64
     assert lib.ss_sweep_autotune_f32(1, arr, 2, 0, None) == -1
     arr[0].geom = _lib.make_geom(2, 3, 20, 30, 8, 50, 1, (0, 15, 29))
     arr[0].tile_h = arr[0].tile_w = arr[0].depth = 0
