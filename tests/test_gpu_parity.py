@@ -159,6 +159,10 @@ def test_mute_weights_match_oracle(dtype):
     want = fd.OracleSim(dtype=dtype).mute_weights(vm).numpy()
     np.testing.assert_allclose(got, want, rtol=0, atol=4 * np.finfo(np.float32 if dtype == torch.float32
                                                                       else np.float64).eps)
+    # before and after the 30 ms taper the weights are exactly 0 and 1, as in the oracle
+    ends = (want == 0) | (want == 1)
+    assert ends.mean() > 0.9
+    np.testing.assert_array_equal(got[ends], want[ends])
 
 
 def test_calibrate_matches_reference(golden):
