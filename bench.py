@@ -254,7 +254,8 @@ def run_autograd(args):
         'vs_baseline': None, 'dtype': 'f32', 'data': 'synthetic (seeded mixed-structure models)',
         'config': {'workload': 'differentiation: misfit gradient of 16 mixed 70x70 models x 5 shots, nt 1000, '
                                'autograd.Function, checkpoints every 50 steps with recompute',
-                   'models': len(true), 'seed': args.seed, 'misfit': loss},
+                   'models': len(true), 'seed': args.seed, 'misfit': loss,
+                   'segment_tiles': {('forward', 'adjoint')[d]: list(t) for d, t in sorted(sim._seg_tiles.items())}},
         'roofline': {'bound': 'hbm', 'kernel': dominant, 'achieved': alg / (per[dominant] * 1e-3) / 1e9,
                      'peak': PEAK_HBM_GBS, 'unit': 'GB/s', 'frac': alg / (per[dominant] * 1e-3) / 1e9 / PEAK_HBM_GBS,
                      'traffic': None, 'kernel_ms_per_step': per, 'algorithmic_bytes_per_direction': alg,

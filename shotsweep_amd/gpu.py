@@ -201,6 +201,19 @@ class Sim:
             out.zero_()
         return out
 
+    def _stable(self, name, t):
+        """t's values in a cached workspace of the same shape.  The sweep entry points
+        replay hipGraphs keyed by buffer addresses, so tensors made per call are
+        copied into buffers that keep their address from call to call."""
+        buf = self._scratch(name, tuple(t.shape))
+        if buf.data_ptr() != t.data_ptr():
+            buf.copy_(t)
+        return buf
+
+    def _stable_prep(self, prep):
+        return dict(prep, c2=self._stable('seg_c2', prep['c2']), amp=self._stable('seg_amp', prep['amp']),
+                    sd=self._stable('seg_sd', prep['sd']))
+
     def release(self):
         """Free cached device workspaces."""
         self._ws.clear()
@@ -485,6 +498,8 @@ class Sim:
         nb = prep['nb']
         H, W = self.H, self.W
         plane = nb * g.ns * H * W
+        prep = self._stable_prep(prep)
+        gathers = self._scratch('seg_gathers', tuple(sim.shape))
         fst = self._scratch('seg_fstate', (4 * plane,))
         states = fst.view(2, 2, nb, g.ns, H, W)        # buffer, (p0, p1), rows, shots
         lib = _lib.load()
@@ -500,10 +515,12 @@ class Sim:
                 ck[:, :, k].copy_(states[phase].permute(1, 2, 0, 3, 4))
             end = self._timed('forward')
             _lib.check(lib.ss_sweep_forward_f32(1, self._sweep_group(prep, 0, t0, t1, k > 0, phase, fst, None,
-                                                                     gathers=sim), 1, st), 'sweep forward')
+                                                                     gathers=gathers), 1, st), 'sweep forward')
             if end is not None:
                 end.record()
             phase = (phase - (-(t1 - t0) // depth)) & 1
+        if gathers.data_ptr() != sim.data_ptr():
+            sim.copy_(gathers)
 
     def _segments_tiled(self, prep, tape, res, gacc, illum, sg):
         """Checkpointed adjoint on the tiled kernels (gpu.py:169-184), for batches too
@@ -519,6 +536,11 @@ class Sim:
         ck, seg = tape['ckpt'], tape['seg']
         nseg = ck.shape[2]
         plane = nb * g.ns * H * W
+        prep = self._stable_prep(prep)
+        res = self._stable('seg_res', res)
+        illum_out, sg_out = illum, sg
+        illum = self._scratch('seg_illum', tuple(illum.shape))   # the first adjoint call overwrites them
+        sg = self._scratch('seg_sg', tuple(sg.shape))
         fst = self._scratch('seg_fstate', (4 * plane,))     # forward: 2 buffers x (p0, p1)
         ast = self._scratch('seg_astate', (4 * plane,))     # adjoint: 2 buffers x (m1, m2)
         hist = self._scratch('hist_seg0', (nb, g.ns, seg, H, W))
@@ -545,6 +567,8 @@ class Sim:
             if end is not None:
                 end.record()
             phase = (phase - (-(t1 - t0) // self._seg_tiles[1][2])) & 1
+        illum_out.copy_(illum)
+        sg_out.copy_(sg)
 
     def _gradient_device(self, prep, observed=None, obs_repeat=1, v_mute=None, t_mute=0.12, res=None):
         """Forward, muted residual, exact adjoint and fold on the device.

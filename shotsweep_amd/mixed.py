@@ -126,6 +126,17 @@ class MixedSweep:
         if end is not None:
             end.record()
 
+    def _stable_prep(self, k, prep):
+        """prep with c2, amp and the source derivative copied into this group's cached
+        buffers: the sweeps replay hipGraphs keyed by buffer addresses, and _prepare
+        makes new tensors on every call."""
+        out = dict(prep)
+        for key in ('c2', 'amp', 'sd'):
+            buf = self._get(k, 'prep_' + key, tuple(prep[key].shape))
+            buf.copy_(prep[key])
+            out[key] = buf
+        return out
+
     def _buffers(self, k, nb):
         sim = self.sims[k]
         g = sim.geom
@@ -152,7 +163,7 @@ class MixedSweep:
         for k, b in enumerate(batches):
             if b is None:
                 continue
-            parts.append((k, b['prep'], self._buffers(k, b['prep']['nb'])))
+            parts.append((k, self._stable_prep(k, b['prep']), self._buffers(k, b['prep']['nb'])))
         if not parts:
             raise ValueError('no rows to sweep')
         self._run(parts, FORWARD)
