@@ -260,7 +260,9 @@ def run_autograd(args):
                      'peak': PEAK_HBM_GBS, 'unit': 'GB/s', 'frac': alg / (per[dominant] * 1e-3) / 1e9 / PEAK_HBM_GBS,
                      'traffic': None, 'kernel_ms_per_step': per, 'algorithmic_bytes_per_direction': alg,
                      'note': 'forward time includes the per-segment recompute (two forward passes); kernel '
-                             'times from one extra step with events around every launch'},
+                             'times from one extra step with events around every launch, in which the '
+                             'segment recomputes run on the caller stream (the timed steps run each recompute '
+                             'on a second stream beside the previous segment\'s adjoint)'},
     }
     if not args.no_cpu:
         from oracle import fd

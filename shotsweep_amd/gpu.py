@@ -201,6 +201,16 @@ class Sim:
             out.zero_()
         return out
 
+    def _side_stream(self, main):
+        """Stream of the segment recomputes: a second stream beside the caller's, or the
+        caller's own while HIP events are recorded (each event interval then spans one
+        launch alone)."""
+        if self.events is not None:
+            return main
+        if self._side is None:
+            self._side = torch.cuda.Stream(device=self.device)  # default priority
+        return self._side
+
     def _stable(self, name, t):
         """t's values in a cached workspace of the same shape.  The sweep entry points
         replay hipGraphs keyed by buffer addresses, so tensors made per call are
@@ -406,9 +416,7 @@ class Sim:
         ck, seg = tape['ckpt'], tape['seg']
         nseg = ck.shape[2]
         main = torch.cuda.current_stream(self.device)
-        if self._side is None:
-            self._side = torch.cuda.Stream(device=self.device)  # default priority
-        side = self._side
+        side = self._side_stream(main)
         bufs = [(self._scratch(f'p0_{i}', (nb, g.ns, H, W)), self._scratch(f'p1_{i}', (nb, g.ns, H, W)),
                  self._scratch(f'hist_seg{i}', (nb, g.ns, seg, H, W))) for i in (0, 1)]
         freed = [None, None]        # event after the adjoint that last read segment buffer i
@@ -528,7 +536,10 @@ class Sim:
         first, ss_sweep_forward_f32 recomputes [t0, t1) from the checkpoint
         (resume) and keeps the Laplacians, then ss_sweep_adjoint_f32 runs [t0, t1)
         backwards from the previous segment's adjoint state, every propagation
-        split into tiles over all CUs.  Per cell the arithmetic is that of the
+        split into tiles over all CUs.  As in the resident loop, the recompute of
+        segment k-1 runs on a side stream beside the adjoint of segment k (two
+        alternating history buffers), which fills the gaps between the short
+        dependent launches of each.  Per cell the arithmetic is that of the
         resident path, so the results are the same bit for bit."""
         g = self.geom
         nb = prep['nb']
@@ -543,30 +554,58 @@ class Sim:
         sg = self._scratch('seg_sg', tuple(sg.shape))
         fst = self._scratch('seg_fstate', (4 * plane,))     # forward: 2 buffers x (p0, p1)
         ast = self._scratch('seg_astate', (4 * plane,))     # adjoint: 2 buffers x (m1, m2)
-        hist = self._scratch('hist_seg0', (nb, g.ns, seg, H, W))
+        hists = [self._scratch(f'hist_seg{i}', (nb, g.ns, seg, H, W)) for i in (0, 1)]
         lib = _lib.load()
-        st = self._stream()
+        main = torch.cuda.current_stream(self.device)
+        side = self._side_stream(main)
+        freed = [None, None]        # event after the adjoint that last read history buffer i
+        ready = {}
+
+        def span(k):
+            return k * seg, min(g.nt, (k + 1) * seg)
+
+        def view(k):
+            t0, t1 = span(k)
+            return hists[k % 2].reshape(-1)[:nb * g.ns * (t1 - t0) * H * W]
+
+        def recompute(k):
+            t0, t1 = span(k)
+            if freed[k % 2] is not None:
+                side.wait_event(freed[k % 2])
+            with torch.cuda.stream(side):
+                fst[:2 * plane].view(2, nb, g.ns, H, W).copy_(ck[:, :, k].permute(2, 0, 1, 3, 4))
+                end = self._timed('forward')
+                _lib.check(lib.ss_sweep_forward_f32(1, self._sweep_group(prep, 0, t0, t1, 1, 0, fst, view(k)), 1,
+                                                    self._stream()), 'sweep forward')
+                if end is not None:
+                    end.record()
+                ready[k] = torch.cuda.Event()
+                ready[k].record(side)
+
+        self._seg_tune(prep, 0, *span(nseg - 1), fst, hists[0])   # forward trials overwrite fst: tune first
+        side.wait_stream(main)      # checkpoints, seed and zeroed state are ready
+        recompute(nseg - 1)
+        if 1 not in self._seg_tiles:
+            # adjoint trials read the last segment's history; the first real adjoint call
+            # (resume 0) then overwrites the outputs the trials left
+            main.wait_event(ready[nseg - 1])
+            self._seg_tune(prep, 1, *span(nseg - 1), ast, view(nseg - 1), res, gacc, illum, sg)
         phase = 0                   # adjoint state buffer that holds (m1, m2)
         for k in reversed(range(nseg)):
-            t0, t1 = k * seg, min(g.nt, (k + 1) * seg)
-            h = hist.reshape(-1)[:nb * g.ns * (t1 - t0) * H * W]
-            self._seg_tune(prep, 0, t0, t1, fst, hist)   # forward trials overwrite the segment buffers: tune first
-            fst[:2 * plane].view(2, nb, g.ns, H, W).copy_(ck[:, :, k].permute(2, 0, 1, 3, 4))
-            end = self._timed('forward')
-            _lib.check(lib.ss_sweep_forward_f32(1, self._sweep_group(prep, 0, t0, t1, 1, 0, fst, h), 1, st),
-                       'sweep forward')
-            if end is not None:
-                end.record()
-            if 1 not in self._seg_tiles:
-                # adjoint trials read this segment's history (valid now); the first real adjoint
-                # call (resume 0) then overwrites the outputs the trials left
-                self._seg_tune(prep, 1, t0, t1, ast, hist, res, gacc, illum, sg)
+            if k > 0:
+                recompute(k - 1)
+            main.wait_event(ready.pop(k))
+            t0, t1 = span(k)
             end = self._timed('adjoint')
-            _lib.check(lib.ss_sweep_adjoint_f32(1, self._sweep_group(prep, 1, t0, t1, k < nseg - 1, phase, ast, h,
-                                                                    res, gacc, illum, sg), 1, st), 'sweep adjoint')
+            _lib.check(lib.ss_sweep_adjoint_f32(1, self._sweep_group(prep, 1, t0, t1, k < nseg - 1, phase, ast,
+                                                                    view(k), res, gacc, illum, sg), 1,
+                                                self._stream()), 'sweep adjoint')
             if end is not None:
                 end.record()
+            freed[k % 2] = torch.cuda.Event()
+            freed[k % 2].record(main)
             phase = (phase - (-(t1 - t0) // self._seg_tiles[1][2])) & 1
+        main.wait_stream(side)      # later allocations on the caller's stream come after the side work
         illum_out.copy_(illum)
         sg_out.copy_(sg)
 
