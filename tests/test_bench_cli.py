@@ -13,3 +13,27 @@ def test_bench_refuses_more_than_one_gpu():
     r = subprocess.run([sys.executable, str(ROOT / 'bench.py'), '--gpus', '2', '--steps', '1'],
                        cwd=ROOT, env=env, capture_output=True, text=True, timeout=300)
     assert r.returncode == 2 and 'measures one GPU' in r.stderr and r.stdout == ''
+
+
+def test_cpu_baseline_is_median_of_three_timed_runs():
+    """bench.cpu_baseline (the oracle on the host) on one model: a warm-up, three
+    timed runs over thirds of the sample, the median rate, the sample's images
+    in row order, and the FP64 oracle's images of the first model."""
+    import numpy as np
+    sys.path.insert(0, str(ROOT))
+    import bench
+    from oracle import fd
+    from shotsweep_amd import background as BG, synth
+    models = synth.mixed(1, seed=3)
+    obs = fd.OracleSim().gathers(models)
+    trend = BG.depth_trend(models)
+    bgs = BG.backgrounds(float(models[0, 0].mean()), trend)
+    vm = np.repeat(models[:, 0].mean(1), 3)
+    base, img, illum, img64 = bench.cpu_baseline(bgs, obs, vm, 3)
+    assert base['kind'] == 'port' and base['unit'] == 'shot-images/s' and base['cores'] >= 1
+    assert base['value'] > 0 and 'median' in base['sample'] and '3 background rows' in base['sample']
+    assert img.shape == illum.shape == (3, 5, 70, 70) and img64.shape == (3, 5, 70, 70)
+    want, _ = fd.OracleSim().images(bgs, np.repeat(obs, 3, 0), v_mute=vm)
+    np.testing.assert_array_equal(img, want)       # thirds of the sample, concatenated in order
+    e = np.linalg.norm(img.astype(np.float64) - img64) / np.linalg.norm(img64)
+    assert 0 < e < 1e-4                             # FP32 against FP64 rounding, not a different result
