@@ -227,6 +227,24 @@ def test_edge_geometries_vs_oracle(nz, nx, ns):
     assert rel(gg, og) < F32_TOL
 
 
+@pytest.mark.parametrize('laplacian,eps,mute', [(False, 1e-3, False), (True, 1e-2, True), (False, 1e-4, True)])
+def test_image_options_vs_oracle(laplacian, eps, mute):
+    """images() with and without the Laplacian filter, other illumination
+    regularisations and without a mute (gpu.py:196-210): illumination bit-identical
+    to the oracle, images within 1e-5."""
+    case(f'image options laplacian={laplacian} eps={eps} mute={mute} 24x28 nt=260')
+    rng = np.random.default_rng(8)
+    v = 1800 + 900 * rng.random((2, 24, 28))
+    obs = fd.OracleSim(pad=12, geom=ORACLE_SMALL).gathers(v) * 0.9
+    vm = np.array([1900.0, 2100.0]) if mute else None
+    oi, oil = fd.OracleSim(pad=12, geom=ORACLE_SMALL).images(v, obs, v_mute=vm, t_mute=0.05, eps=eps,
+                                                               laplacian=laplacian)
+    gi, gil = gpu.Sim(device='cuda', pad=12, geom=SMALL).images(v, obs, v_mute=vm, t_mute=0.05, eps=eps,
+                                                                laplacian=laplacian)
+    np.testing.assert_array_equal(gil, oil)
+    assert np.abs(oi).max() > 0 and rel(gi, oi) < F32_TOL
+
+
 def test_dot_product_fp64():
     case('dot product <y,Fx> = <F*y,x> FP64')
     sim = gpu.Sim(device='cuda', dtype=torch.float64, pad=12, geom=SMALL)
