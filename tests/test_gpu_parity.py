@@ -178,6 +178,7 @@ def test_mixed_models_resident_vs_oracle():
     osim = fd.OracleSim()
     obs = sim.gathers(models)
     np.testing.assert_array_equal(obs, osim.gathers(models))
+    np.testing.assert_array_equal(sim.gathers(models, chunk=1), obs)   # chunking (gpu.py:138-140)
     trend = np.linspace(0, 1500, 70)
     bgs = np.concatenate([np.stack([np.tile(np.clip(m[0].mean() + r * trend, 1500, 4500)[:, None], (1, 70))
                                     for r in (0.6, 1.0, 1.4)]) for m in models])
