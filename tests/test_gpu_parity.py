@@ -246,6 +246,22 @@ def test_image_options_vs_oracle(laplacian, eps, mute):
     assert np.abs(oi).max() > 0 and rel(gi, oi) < F32_TOL
 
 
+def test_forward_checkpoints_vs_oracle():
+    """forward(v, store=True) (gpu.py:117-136): gathers, the (p0, p1) checkpoints
+    before every checkpoint_interval-th step (interval 73, a short last one) and
+    c2 bit-identical to the oracle's."""
+    case('forward store=True checkpoints 24x28 nt=260 interval 73 vs oracle')
+    v = 1800 + 900 * np.random.default_rng(9).random((2, 24, 28))
+    got = gpu.Sim(device='cuda', pad=12, geom=SMALL, checkpoint_interval=73).forward(v, store=True)
+    want = fd.OracleSim(pad=12, geom=ORACLE_SMALL, checkpoint_interval=73).forward(v, store=True)
+    np.testing.assert_array_equal(got[0].cpu().numpy(), want[0].numpy())
+    assert sorted(got[1]) == sorted(want[1]) == [0, 73, 146, 219]
+    for it, (p0, p1) in want[1].items():
+        np.testing.assert_array_equal(got[1][it][0].cpu().numpy(), p0.numpy(), err_msg=f'p0 at {it}')
+        np.testing.assert_array_equal(got[1][it][1].cpu().numpy(), p1.numpy(), err_msg=f'p1 at {it}')
+    np.testing.assert_array_equal(got[2].cpu().numpy().reshape(want[2].shape), want[2].numpy())
+
+
 def test_dot_product_fp64():
     case('dot product <y,Fx> = <F*y,x> FP64')
     sim = gpu.Sim(device='cuda', dtype=torch.float64, pad=12, geom=SMALL)
