@@ -29,8 +29,9 @@
 // owns cell pairs (2q, 2q+1), reads both cells' taps with 8-byte LDS accesses
 // and moves history, c2 and residual two cells per buffer instruction; their
 // descriptors hold the sponge-table byte offset directly, and the table sits
-// at LDS address 0 ((A, B) entries for the forward, (A, 2A, A*B, 0) for the
-// adjoint).  Other shapes run the cell kernels with runtime dimensions.
+// at LDS address 0 ((A, B) entries for the forward, (2A, A*B) for the adjoint,
+// A = 0.5 * 2A exactly).  Other shapes run the cell kernels with runtime
+// dimensions.
 #include <algorithm>
 
 #include "ss_device.h"
@@ -88,12 +89,11 @@ __device__ __forceinline__ uint32_t cell_desc(int i, int H, int W, int nz, int n
 }
 
 // Cell descriptor of the paired kernels (16 bits): bit 0 Laplacian interior,
-// bit 1 model cell, and the byte offset of the cell's sponge-table entry: bits
-// 3-13 for the forward's 8-byte (A, B) entries, bits 4-14 for the adjoint's
-// 16-byte (A, 2A, A B, 0) entries.  Model cells sit at sponge depth 0, i.e.
-// entry 0 (A = B = 1).  The table starts at LDS address 0, so a lookup is one
-// mask and the load itself.
-constexpr uint32_t kPInterior = 1u, kPModel = 2u, kPTabMask = 0x3ff8u, kPTab4Mask = 0x7ff0u;
+// bit 1 model cell, and in bits 3-13 the byte offset of the cell's 8-byte
+// sponge-table entry, (A, B) for the forward and (2A, A B) for the adjoint.
+// Model cells sit at sponge depth 0, i.e. entry 0 (A = B = 1).  The table
+// starts at LDS address 0, so a lookup is one mask and the load itself.
+constexpr uint32_t kPInterior = 1u, kPModel = 2u, kPTabMask = 0x3ff8u;
 template <int SHIFT>
 __device__ __forceinline__ uint32_t pair_cell_desc(int i, int H, int W, int nz, int nx, int pad) {
   const int z = i / W, x = i - z * W;
@@ -103,8 +103,6 @@ __device__ __forceinline__ uint32_t pair_cell_desc(int i, int H, int W, int nz, 
   return v;
 }
 __device__ __forceinline__ float2 sponge_at(uint32_t d) { return lds_table2(d & kPTabMask); }
-__device__ __forceinline__ float3 sponge4_at(uint32_t d) { return lds_table3(d & kPTab4Mask); }
-__device__ __forceinline__ float sponge4_a(uint32_t d) { return lds_table_a(d & kPTab4Mask); }
 
 // Per-cell descriptors are loop-invariant; left visible, the compiler hoists
 // every value derived from them (table addresses, flag masks) out of the time
@@ -137,15 +135,19 @@ __device__ void fill_table(float2 *tab, const float *A, const float *B, int W, i
 }
 
 // Table (dz, dx) -> (A, 2A, A*B, 0) of the paired adjoint (2A is exact).
-__device__ void fill_table4(float4 *tab, const float *A, const float *B, int W, int pad) {
+// Adjoint table (2A, A*B) in 8-byte entries; A itself is 0.5 * 2A, exact (A is a
+// normal number in (0, 1]), so one 8-byte read gives all three coefficients.
+__device__ void fill_table2a(float2 *tab, const float *A, const float *B, int W, int pad) {
   const int n = (pad + 1) * (pad + 1);
   for (int k = threadIdx.x; k < n; k += blockDim.x) {
     const int dz = k / (pad + 1), dx = k - dz * (pad + 1);
     const int c = (pad - dz) * W + (pad - dx);
     const float a = A[c], b = B[c];
-    tab[k] = table4_entry(a, b);
+    tab[k] = make_float2(2.0f * a, a * b);
   }
 }
+__device__ __forceinline__ float2 sponge2a_at(uint32_t d) { return lds_table2(d & kPTabMask); }
+__device__ __forceinline__ float sponge2a_a(uint32_t d) { return 0.5f * lds_table_a(d & kPTabMask); }
 
 // Workgroup -> propagation.  Workgroups are dealt round-robin to the 8 XCDs
 // (blockIdx % 8), so XCD x receives blocks x, x+8, ...; mapping those to the
@@ -181,7 +183,7 @@ size_t fwd_pair_lds_bytes(const Geom &g) {
   return (size_t)(g.pad + 1) * (g.pad + 1) * sizeof(float2) + (size_t)(g.H * g.W + 2 * pair_guard(g.W)) * sizeof(float);
 }
 size_t adj_pair_lds_bytes(const Geom &g) {
-  return (size_t)(g.pad + 1) * (g.pad + 1) * sizeof(float4) +
+  return (size_t)(g.pad + 1) * (g.pad + 1) * sizeof(float2) +
          (size_t)(g.H * g.W + 2 * pair_guard(g.W) + g.nz * g.nx) * sizeof(float);
 }
 
@@ -328,7 +330,7 @@ __global__ __launch_bounds__(kFwdThreads) void fwd_pair_kernel(Geom g, int t0, i
   extern __shared__ float lds[];
   constexpr int nz = NZ, nx = NX, pad = PAD, row = ROW;
   constexpr int H = nz + 2 * pad, W = nx + 2 * pad;
-  constexpr int HW = H * W, NP = HW / 2, W2 = W / 2;
+  constexpr int HW = H * W, NP = HW / 2;
   static_assert(W % 2 == 0 && HW % 2 == 0, "pairs need an even plane width");
   static_assert((pad + 1) * (pad + 1) * 8 <= (int)kPTabMask + 8, "sponge-table offsets must fit the descriptor");
   constexpr int guard = pair_guard(W), tabf = (pad + 1) * (pad + 1) * 2;
@@ -631,11 +633,11 @@ __global__ __launch_bounds__(kAdjThreads) void adj_pair_kernel(Geom g, int t0, i
   extern __shared__ float lds[];
   constexpr int nz = NZ, nx = NX, pad = PAD, row = ROW;
   constexpr int H = nz + 2 * pad, W = nx + 2 * pad;
-  constexpr int HW = H * W, NP = HW / 2, W2 = W / 2;
+  constexpr int HW = H * W, NP = HW / 2;
   static_assert(W % 2 == 0 && nx % 2 == 0 && pad % 2 == 0, "pairs need even widths");
-  static_assert((pad + 1) * (pad + 1) * 16 <= (int)kPTab4Mask + 16, "sponge-table offsets must fit the descriptor");
-  constexpr int guard = pair_guard(W), tabf = (pad + 1) * (pad + 1) * 4;
-  float4 *tab = reinterpret_cast<float4 *>(lds);  // at LDS address 0: lookups need no base add
+  static_assert((pad + 1) * (pad + 1) * 8 <= (int)kPTabMask + 8, "sponge-table offsets must fit the descriptor");
+  constexpr int guard = pair_guard(W), tabf = (pad + 1) * (pad + 1) * 2;
+  float2 *tab = reinterpret_cast<float2 *>(lds);  // at LDS address 0: lookups need no base add
   float *Q = lds + tabf + guard;
   float2 *Q2 = reinterpret_cast<float2 *>(Q);
   constexpr uint32_t kQAddr = (uint32_t)(tabf + guard) * 4;  // LDS byte address of Q
@@ -650,7 +652,7 @@ __global__ __launch_bounds__(kAdjThreads) void adj_pair_kernel(Geom g, int t0, i
     Q[k - guard] = 0.0f;
     Q[HW + k] = 0.0f;
   }
-  fill_table4(tab, A, B, W, pad);
+  fill_table2a(tab, A, B, W, pad);
   for (int k = t; k < nzx; k += kAdjThreads) il[k] = illumg[prop * nzx + k];
   __syncthreads();
   uint32_t pk[KP];
@@ -659,8 +661,8 @@ __global__ __launch_bounds__(kAdjThreads) void adj_pair_kernel(Geom g, int t0, i
     const int q = t + j * kAdjThreads;
     // bit 31 (the second cell's unused bit 15) repeats the model flag, so one
     // signed compare tests it
-    const uint32_t d0 = pair_cell_desc<4>(2 * q, H, W, nz, nx, pad);
-    pk[j] = q < NP ? (d0 | (pair_cell_desc<4>(2 * q + 1, H, W, nz, nx, pad) << 16) | ((d0 & kPModel) ? 0x80000000u : 0u))
+    const uint32_t d0 = pair_cell_desc<3>(2 * q, H, W, nz, nx, pad);
+    pk[j] = q < NP ? (d0 | (pair_cell_desc<3>(2 * q + 1, H, W, nz, nx, pad) << 16) | ((d0 & kPModel) ? 0x80000000u : 0u))
                    : 0u;
   }
   const __amdgpu_buffer_rsrc_t srs1 = make_rsrc(m1g + prop * HW, HW);
@@ -683,8 +685,8 @@ __global__ __launch_bounds__(kAdjThreads) void adj_pair_kernel(Geom g, int t0, i
     if (q < NP) {
       const uint32_t d0 = pk[j] & 0xffffu, d1 = pk[j] >> 16;
       const float2 cv = buf_load2(crs, t * 8, j * kAdjThreads * 8);
-      Q2[q] = make_float2((d0 & kPInterior) ? div12((cv.x * sponge4_a(d0)) * m1[j].x) : 0.0f,
-                          (d1 & kPInterior) ? div12((cv.y * sponge4_a(d1)) * m1[j].y) : 0.0f);
+      Q2[q] = make_float2((d0 & kPInterior) ? div12((cv.x * sponge2a_a(d0)) * m1[j].x) : 0.0f,
+                          (d1 & kPInterior) ? div12((cv.y * sponge2a_a(d1)) * m1[j].y) : 0.0f);
     }
   }
   constexpr int rec0 = (pad + row) * W + pad, rec1 = rec0 + nx;
@@ -736,13 +738,13 @@ __global__ __launch_bounds__(kAdjThreads) void adj_pair_kernel(Geom g, int t0, i
         const float2 u2p = lds_ld2(a + 16 * W);
         const float lt0 = lapt_sum(r1.x, u2p.x, up.x, um.x, u2m.x, r2.x, r1.y, r0.y, r0.x);
         const float lt1 = lapt_sum(r1.y, u2p.y, up.y, um.y, u2m.y, r2.y, r2.x, r1.x, r0.y);
-        const float3 aab0 = sponge4_at(d0), aab1 = sponge4_at(d1);  // (A, 2A, A B)
-        float mu0 = aab0.y * m1[j].x;
-        float mu1 = aab1.y * m1[j].y;
+        const float2 e0 = sponge2a_at(d0), e1 = sponge2a_at(d1);  // (2A, A B)
+        float mu0 = e0.x * m1[j].x;
+        float mu1 = e1.x * m1[j].y;
         mu0 = mu0 + lt0;
         mu1 = mu1 + lt1;
-        mu0 = mu0 - aab0.z * m2[j].x;
-        mu1 = mu1 - aab1.z * m2[j].y;
+        mu0 = mu0 - e0.y * m2[j].x;
+        mu1 = mu1 - e1.y * m2[j].y;
         if (j * kAdjThreads < rec1 / 2 && (j + 1) * kAdjThreads > rec0 / 2) {
           if (q >= rec0 / 2 && q < rec1 / 2) {
             const float2 rv = buf_load2(rrs, (2 * q - rec0) * 4, it * nx * 4);
@@ -750,8 +752,8 @@ __global__ __launch_bounds__(kAdjThreads) void adj_pair_kernel(Geom g, int t0, i
             mu1 = mu1 + rv.y;
           }
         }
-        ga[j].x = ga[j].x + (aab0.x * mu0) * L.x;
-        ga[j].y = ga[j].y + (aab1.x * mu1) * L.y;
+        ga[j].x = ga[j].x + ((0.5f * e0.x) * mu0) * L.x;  // A = 0.5 (2A), exact
+        ga[j].y = ga[j].y + ((0.5f * e1.x) * mu1) * L.y;
         if (j == srcq / kAdjThreads) {
           if (q == srcq) sg = sg + (((srci & 1) ? mu1 : mu0) * wt) * sdv;
         }
@@ -782,7 +784,7 @@ __global__ __launch_bounds__(kAdjThreads) void adj_pair_kernel(Geom g, int t0, i
       }
       if (t + j * kAdjThreads < NP) {
         const uint32_t d0 = pk[j] & 0xffffu, d1 = pk[j] >> 16;
-        const float2 qq = div12x2((cv.x * sponge4_a(d0)) * m1[j].x, (cv.y * sponge4_a(d1)) * m1[j].y);
+        const float2 qq = div12x2((cv.x * sponge2a_a(d0)) * m1[j].x, (cv.y * sponge2a_a(d1)) * m1[j].y);
         // both cells of a pair share the interior flag (bit 0)
         Q2[off + j * kAdjThreads] = make_float2(keep_if<0>(qq.x, pk[j]), keep_if<0>(qq.y, pk[j]));
       }
