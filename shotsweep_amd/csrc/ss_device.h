@@ -51,8 +51,8 @@ __device__ __forceinline__ float2 buf_load2(__amdgpu_buffer_rsrc_t r, int voff, 
 // Sponge tables at LDS address 0.  Kernels without static LDS (checked on the
 // host before launch) start their dynamic LDS at address 0, so a table placed
 // first is read at the LDS address equal to the entry's byte offset, with no
-// base add.  Forward tables hold (A, B) in 8-byte entries, adjoint tables
-// (A, 2A, A*B, 0) in 16-byte entries (2A is exact).
+// base add.  Tables hold 8-byte entries: (A, B) for the forward, (A, A*B) or
+// (2A, A*B) for the adjoint (2A is exact, and A = 0.5 * 2A).
 typedef float ss_v2f __attribute__((ext_vector_type(2)));
 typedef __attribute__((address_space(3))) const float ss_lds_cfloat;
 typedef __attribute__((address_space(3))) const ss_v2f ss_lds_cv2f;
@@ -62,15 +62,9 @@ __device__ __forceinline__ float2 lds_table2(uint32_t byte_off) {
 }
 // 8-byte read at an LDS byte address (same addressing as the tables).
 __device__ __forceinline__ float2 lds_ld2(uint32_t byte_addr) { return lds_table2(byte_addr); }
-// (A, 2A, A*B) of a 16-byte adjoint entry
-__device__ __forceinline__ float3 lds_table3(uint32_t byte_off) {
-  const ss_v2f v = *reinterpret_cast<ss_lds_cv2f *>((uintptr_t)byte_off);
-  return make_float3(v.x, v.y, *reinterpret_cast<ss_lds_cfloat *>((uintptr_t)byte_off + 8));
-}
 __device__ __forceinline__ float lds_table_a(uint32_t byte_off) {
   return *reinterpret_cast<ss_lds_cfloat *>((uintptr_t)byte_off);
 }
-__device__ __forceinline__ float4 table4_entry(float a, float b) { return make_float4(a, 2.0f * a, a * b, 0.0f); }
 
 // v if bit BIT of d is set, else +0: the sign-extended bit as an AND mask.  The
 // empty asm keeps the compiler from turning the AND back into compare + select.

@@ -134,8 +134,7 @@ __device__ void fill_table(float2 *tab, const float *A, const float *B, int W, i
   }
 }
 
-// Table (dz, dx) -> (A, 2A, A*B, 0) of the paired adjoint (2A is exact).
-// Adjoint table (2A, A*B) in 8-byte entries; A itself is 0.5 * 2A, exact (A is a
+// Adjoint table (dz, dx) -> (2A, A*B) in 8-byte entries; A itself is 0.5 * 2A, exact (A is a
 // normal number in (0, 1]), so one 8-byte read gives all three coefficients.
 __device__ void fill_table2a(float2 *tab, const float *A, const float *B, int W, int pad) {
   const int n = (pad + 1) * (pad + 1);

@@ -75,8 +75,8 @@ struct SGroup {
 };
 
 // LDS layout (floats): [sponge table | guard | plane 0 | guard | plane 1 | guard | illumination].
-// The table starts at LDS address 0 (tab_off 0): 8-byte (A, B) or (A, A*B)
-// entries, or the paired adjoint's 16-byte (A, 2A, A*B, 0) entries.
+// The table starts at LDS address 0 (tab_off 0): 8-byte (A, B), (A, A*B) or,
+// for the paired adjoint, (2A, A*B) entries.
 struct SArgs {
   SGroup g[kMaxGroups];
   int ngroups, k, nactive;
@@ -383,14 +383,17 @@ __global__ __launch_bounds__(kThreads) void sweep_adj_kernel(SArgs a) {
 // per cell), B = the two cells' sponge-table byte offsets.  Per cell the
 // arithmetic is that of the cell kernels above.
 constexpr uint32_t kQInt = 1u << 17, kQOwn = 1u << 18, kQRow = 1u << 19, kQOwnModel = 1u << 20;
-constexpr uint32_t kTabOff = 0x3ff8u, kTab4Off = 0x7ff0u;  // table byte offsets: 8-byte / 16-byte entries
+constexpr uint32_t kTabOff = 0x3ff8u;  // table byte offsets of 8-byte entries
 
-__device__ __forceinline__ void sweep_table4(float4 *tab, const SGroup &G) {
+// Adjoint table of the paired kernel: (2A, A*B); A = 0.5 * 2A exactly (A is a
+// normal number in (0, 1]), so one 8-byte read gives all three coefficients.
+__device__ __forceinline__ void sweep_table2a(float2 *tab, const SGroup &G) {
   const int n = (G.pad + 1) * (G.pad + 1);
   for (int k = threadIdx.x; k < n; k += kThreads) {
     const int dz = k / (G.pad + 1), dx = k - dz * (G.pad + 1);
     const int c = (G.pad - dz) * G.W + (G.pad - dx);
-    tab[k] = table4_entry(G.A[c], G.B[c]);
+    const float a = G.A[c], b = G.B[c];
+    tab[k] = make_float2(2.0f * a, a * b);
   }
 }
 
@@ -526,11 +529,11 @@ __global__ __launch_bounds__(kThreads) void sweep_adj_pair_kernel(SArgs a) {
   float2 *m1o = reinterpret_cast<float2 *>(G.st + 2 * (ib ^ 1) * PS + (int64_t)v.prop * HW);
   float2 *m2o = reinterpret_cast<float2 *>(G.st + (2 * (ib ^ 1) + 1) * PS + (int64_t)v.prop * HW);
   float *Qc = lds + a.plane0, *Qn = lds + a.plane1;
-  float4 *tab = reinterpret_cast<float4 *>(lds + a.tab_off);
+  float2 *tab = reinterpret_cast<float2 *>(lds + a.tab_off);
   float2 *il = reinterpret_cast<float2 *>(lds + a.il_off);
   const int t = threadIdx.x;
   zero_guards(Qc, Qn, 2 * NPE, EW);
-  sweep_table4(tab, G);
+  sweep_table2a(tab, G);
   __syncthreads();
 
   const uint32_t magic = 0xffffffffu / (uint32_t)EW + 1u;
@@ -565,11 +568,12 @@ __global__ __launch_bounds__(kThreads) void sweep_adj_pair_kernel(SArgs a) {
       if (z == r && x >= pad && x < pad + G.nx) d |= kQRow;
       da[j] = d;
       const int sz = sponge_dist(z, H, pad) * (pad + 1);
-      const uint32_t o0 = (uint32_t)(sz + sponge_dist(x, W, pad)) << 4;
-      const uint32_t o1 = (uint32_t)(sz + sponge_dist(x + 1, W, pad)) << 4;
+      const uint32_t o0 = (uint32_t)(sz + sponge_dist(x, W, pad)) << 3;
+      const uint32_t o1 = (uint32_t)(sz + sponge_dist(x + 1, W, pad)) << 3;
       db[j] = o0 | (o1 << 16);
       const float2 cv = c2b[gi >> 1];
-      ca[j] = make_float2(cv.x * lds_table_a(o0), cv.y * lds_table_a(o1));  // c2 * A (gpu.py:179)
+      // c2 * A (gpu.py:179), A = 0.5 * 2A
+      ca[j] = make_float2(cv.x * (0.5f * lds_table_a(o0)), cv.y * (0.5f * lds_table_a(o1)));
       float2 ilv = make_float2(0.0f, 0.0f);
       if (a.k > 0 || G.resume) {
         m1[j] = m1i[gi >> 1];
@@ -619,21 +623,21 @@ __global__ __launch_bounds__(kThreads) void sweep_adj_pair_kernel(SArgs a) {
         const float2 um = C2[p - EW2], u2m = C2[p - EW], up = C2[p + EW2], u2p = C2[p + EW];
         const float lt0 = lapt_sum(r1.x, u2p.x, up.x, um.x, u2m.x, r2.x, r1.y, r0.y, r0.x);
         const float lt1 = lapt_sum(r1.y, u2p.y, up.y, um.y, u2m.y, r2.y, r2.x, r1.x, r0.y);
-        const float3 aab0 = lds_table3(db[j] & kTab4Off), aab1 = lds_table3((db[j] >> 16) & kTab4Off);  // (A, 2A, A B)
-        float mu0 = aab0.y * m1[j].x;
-        float mu1 = aab1.y * m1[j].y;
+        const float2 e0 = lds_table2(db[j] & kTabOff), e1 = lds_table2((db[j] >> 16) & kTabOff);  // (2A, A B)
+        float mu0 = e0.x * m1[j].x;
+        float mu1 = e1.x * m1[j].y;
         mu0 = mu0 + lt0;
         mu1 = mu1 + lt1;
-        mu0 = mu0 - aab0.z * m2[j].x;
-        mu1 = mu1 - aab1.z * m2[j].y;
+        mu0 = mu0 - e0.y * m2[j].x;
+        mu1 = mu1 - e1.y * m2[j].y;
         if (d & kQRow) {
           const float2 rv = buf_load2(rrs, (gi - rec0) * 4, it * G.nx * 4);
           mu0 = mu0 + rv.x;
           mu1 = mu1 + rv.y;
         }
         // off the own tile L = 0: the imaging sum there is never stored, the illumination unchanged
-        ga[j].x = ga[j].x + (aab0.x * mu0) * L.x;
-        ga[j].y = ga[j].y + (aab1.x * mu1) * L.y;
+        ga[j].x = ga[j].x + ((0.5f * e0.x) * mu0) * L.x;  // A = 0.5 (2A), exact
+        ga[j].y = ga[j].y + ((0.5f * e1.x) * mu1) * L.y;
         if (d & kQOwnModel) {  // illumination of the own model cells only (the rest is never stored)
           const float2 iv = il[p];
           il[p] = make_float2(iv.x + L.x * L.x, iv.y + L.y * L.y);
@@ -693,7 +697,8 @@ int ext_cells(int H, int W, const TileChoice &c) {
 }
 
 // LDS floats for the largest extended region ehw (widest ew) and sponge width pad.
-// The table comes first (LDS address 0); an adjoint reserves 16-byte entries.
+// The table comes first (LDS address 0); an adjoint reserves 16 bytes per entry
+// (it uses 8), which keeps the tile limits the autotuner ranks against.
 int lds_floats(int ehw, int ew, int pad, int dir, int *plane0, int *plane1, int *il_off) {
   const int guard = 2 * ew + 2;
   const int tab = (dir ? 4 : 2) * (pad + 1) * (pad + 1);
