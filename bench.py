@@ -236,15 +236,13 @@ def run_autograd(args):
         loss, grad = autograd.misfit_and_gradient(start, obs, sim)
     torch.cuda.synchronize()
     wall = (time.perf_counter() - t0) / args.steps
-    # kernel breakdown from one more step with HIP events around every launch (the events
-    # cost host time per launch, so the timed steps above run without them)
-    sim.events = []
+    # kernel breakdown from one more step with HIP events around every launch (the
+    # segment recomputes then run on the caller stream, so each interval is one launch)
+    sim.timing, sim.timing_log = True, []
     autograd.misfit_and_gradient(start, obs, sim)
-    torch.cuda.synchronize()
-    events, sim.events = sim.events, None
-    per = {}
-    for name, a, b in events:
-        per[name] = per.get(name, 0.0) + a.elapsed_time(b)
+    sim.timing = False
+    per = {'forward': sum(t['forward'][1] for t in sim.timing_log),
+           'adjoint': sum(t['adjoint'][1] for t in sim.timing_log)}
     dominant = max(per, key=per.get)
     units = len(true) * ns
     alg = BYTES_PER_CELL_UPDATE * sim.H * sim.W * sim.geom.nt * units
@@ -254,8 +252,7 @@ def run_autograd(args):
         'vs_baseline': None, 'dtype': 'f32', 'data': 'synthetic (seeded mixed-structure models)',
         'config': {'workload': 'differentiation: misfit gradient of 16 mixed 70x70 models x 5 shots, nt 1000, '
                                'autograd.Function, checkpoints every 50 steps with recompute',
-                   'models': len(true), 'seed': args.seed, 'misfit': loss,
-                   'segment_tiles': {('forward', 'adjoint')[d]: list(t) for d, t in sorted(sim._seg_tiles.items())}},
+                   'models': len(true), 'seed': args.seed, 'misfit': loss},
         'roofline': {'bound': 'hbm', 'kernel': dominant, 'achieved': alg / (per[dominant] * 1e-3) / 1e9,
                      'peak': PEAK_HBM_GBS, 'unit': 'GB/s', 'frac': alg / (per[dominant] * 1e-3) / 1e9 / PEAK_HBM_GBS,
                      'traffic': None, 'kernel_ms_per_step': per, 'algorithmic_bytes_per_direction': alg,
@@ -360,19 +357,19 @@ def main():
     for _ in range(args.warmup):
         sim.images_device(prep, obs, 3, vm)
     torch.cuda.synchronize()
-    sim.events = []
+    sim.timing, sim.timing_log = True, []
+    per_kernel = {}
     t0 = time.perf_counter()
     for _ in range(args.steps):
         img, illum = sim.images_device(prep, obs, 3, vm)
+        for name, (n, ms) in sim.timing_log[-1].items():
+            per_kernel.setdefault(name, []).append(ms / max(n, 1))
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    events, sim.events = sim.events, None
+    sim.timing = False
     ms_per_step = wall / args.steps * 1e3
     value = shot_images / (wall / args.steps)
 
-    per_kernel = {}
-    for name, a, b in events:
-        per_kernel.setdefault(name, []).append(a.elapsed_time(b))
     avg = {k: float(np.mean(v)) for k, v in per_kernel.items()}
     dominant = max(avg, key=avg.get)
     H, W, nt = sim.H, sim.W, sim.geom.nt

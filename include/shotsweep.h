@@ -43,9 +43,12 @@ extern "C" {
 #define SS_EUNSUP (-3)   /* requested implementation cannot run this shape */
 
 /* Implementation selector for the time-loop entry points. */
-#define SS_IMPL_AUTO 0      /* resident when the plane fits the LDS, else stepwise */
+#define SS_IMPL_AUTO 0      /* resident when the plane fits the LDS, else stepwise (ss_rtm_*: tiled
+                               checkpoint segments for batches of at most CUs/2 propagations) */
 #define SS_IMPL_STEPWISE 1  /* one launch per time step, planes in HBM (FP32/FP64, any size) */
 #define SS_IMPL_RESIDENT 2  /* one persistent launch per step range, plane in LDS (FP32) */
+#define SS_IMPL_TILED 3     /* ss_rtm_* checkpoint segments on the tiled sweep kernels (FP32) */
+#define SS_IMPL_UNTILED 4   /* ss_rtm_*: as SS_IMPL_AUTO, but never the tiled kernels */
 
 /*
  * Geometry of one batch of propagations (openfwi/physics/constants.py:4-7,
@@ -71,14 +74,15 @@ int ss_version(void);
 int64_t ss_resident_max_cells(void);
 
 /*
- * Device bytes the caller allocates for one gradient or images call on g
- * (the forward with its tape, residual, adjoint and fold), with elements of
- * elem_size bytes (4 or 8): segment >= nt keeps the whole Laplacian history;
- * a smaller segment keeps checkpoints every `segment` steps plus two
- * alternating segment buffers.  This implementation's counterpart of
- * gpu.py:70-73 (imaging_memory_bytes), which estimates the reference's own
- * scheme.  Returns -1 (ss_last_error set) on an invalid geometry or size.
- * Host-only arithmetic: no device is touched.
+ * Bytes of the workspace that ss_rtm_* needs for one gradient or images call
+ * on g (the forward with its tape, residual, adjoint and fold), with elements
+ * of elem_size bytes (4 or 8): segment >= nt keeps the whole Laplacian
+ * history; a smaller segment keeps checkpoints every `segment` steps plus two
+ * alternating segment buffers.  The same size serves every implementation
+ * selector.  This implementation's counterpart of gpu.py:70-73
+ * (imaging_memory_bytes), which estimates the reference's own scheme.
+ * Returns -1 (ss_last_error set) on an invalid geometry or size.  Host-only
+ * arithmetic: no device is touched.
  */
 int64_t ss_workspace_bytes(const ss_geom *g, int segment, int elem_size);
 
@@ -124,6 +128,57 @@ int ss_adjoint_f64(const ss_geom *g, int t0, int t1, const double *c2, const dou
                    const double *w, const double *sd, const double *res, const double *lap_hist,
                    double *m1, double *m2, double *gacc, double *illum, double *src_grad, int impl,
                    void *stream);
+
+/*
+ * One gradient or images call on the device, replacing the body of
+ * Sim._gradient and Sim.images (gpu.py:152-188, 196-210): the forward with its
+ * tape (gpu.py:117-136, 154-157), the muted residual (gpu.py:158-160), the
+ * exact adjoint over the whole Laplacian history or over checkpoint segments
+ * recomputed from their checkpoints (gpu.py:161-184), the fold with the
+ * source-amplitude term (gpu.py:185-187) and, when img is given, the image
+ * epilogue (gpu.py:206-209).  Every intermediate lives in `workspace`
+ * (256-byte aligned, at least ss_workspace_bytes(g, segment, elem size)
+ * bytes); the library allocates no device memory for it.  The two stages may
+ * run in separate calls on one workspace: SS_RTM_FORWARD leaves the tape and
+ * the gathers there, a later SS_RTM_ADJOINT with the same geometry, segment
+ * and impl reads them (an autograd backward passes its data-space gradient
+ * as `seed`).  Checkpoint segments recompute segment k-1 on a side stream
+ * owned by the library (one per thread and device) while segment k's adjoint
+ * runs on `stream`; results do not depend on that overlap.  Calls that share
+ * a workspace must not overlap in time.
+ */
+#define SS_RTM_FORWARD 1
+#define SS_RTM_ADJOINT 2
+
+typedef struct ss_rtm_args {
+  int32_t stages;      /* SS_RTM_FORWARD, SS_RTM_ADJOINT or both */
+  int32_t segment;     /* >= nt: whole Laplacian history; else checkpoints every `segment` steps (gpu.py:131) */
+  int32_t impl;        /* SS_IMPL_* for the time loops */
+  int32_t obs_repeat;  /* row b of the batch uses row b / obs_repeat of obs */
+  int32_t laplacian;   /* image: 1 applies the Laplacian filter (gpu.py:208-209) */
+  int32_t timing;      /* 1: HIP events around every time-loop launch (read with ss_rtm_timing); the
+                          segment recomputes then run on `stream` so each interval is one launch */
+  double dt, dx;       /* time step (s) and grid spacing (m) of the mute (gpu.py:146-149) */
+  double t_mute, ramp; /* mute delay and ramp length (s): 0.12 and 0.03 in the reference */
+  double eps;          /* image: illumination regularisation (gpu.py:206) */
+  double k;            /* (dt/dx)^2, rounded to the working type (gpu.py:185) */
+  const void *c2, *A, *B, *w, *amp; /* as in ss_forward_* */
+  const void *sd;      /* (nb, ns) source-amplitude derivative, as in ss_adjoint_* */
+  const void *v;       /* (nb, nz, nx) velocity of the rows, as in ss_gradient_* */
+  const void *obs;     /* (nb / obs_repeat, ns, nt, nx) observed gathers, or NULL when seed is given */
+  const void *v_mute;  /* (nb) mute velocities, or NULL: no mute */
+  const void *seed;    /* (nb, ns, nt, nx) data-space seed used instead of the residual, or NULL */
+  void *gathers;       /* out (nb, ns, nt, nx) simulated gathers, or NULL */
+  void *grad, *illum;  /* out (nb, ns, nz, nx): per-shot gradient and illumination (adjoint stage) */
+  void *img;           /* out (nb, ns, nz, nx) image, or NULL (adjoint stage) */
+} ss_rtm_args;
+
+int ss_rtm_f32(const ss_geom *g, const ss_rtm_args *a, void *workspace, int64_t workspace_bytes, void *stream);
+int ss_rtm_f64(const ss_geom *g, const ss_rtm_args *a, void *workspace, int64_t workspace_bytes, void *stream);
+
+/* Launch counts and summed durations (ms) of the forward and adjoint time-loop
+ * launches of this thread's last ss_rtm_* call with timing = 1 (waits for them). */
+int ss_rtm_timing(int *nforward, double *forward_ms, int *nadjoint, double *adjoint_ms);
 
 /*
  * Residual with the direct-wave mute, replacing gpu.py:142-150 and 158-160:

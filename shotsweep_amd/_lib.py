@@ -10,8 +10,9 @@ import torch
 
 LIB_PATH = Path(__file__).resolve().parent / 'libshotsweep.so'
 MAX_SHOTS = 32
-IMPL_AUTO, IMPL_STEPWISE, IMPL_RESIDENT = 0, 1, 2
-ABI_VERSION = 5
+IMPL_AUTO, IMPL_STEPWISE, IMPL_RESIDENT, IMPL_TILED, IMPL_UNTILED = 0, 1, 2, 3, 4
+RTM_FORWARD, RTM_ADJOINT = 1, 2
+ABI_VERSION = 6
 
 _vp = ctypes.c_void_p
 _i = ctypes.c_int
@@ -61,6 +62,16 @@ class SSSweepGroup(ctypes.Structure):
                 ('t0', ctypes.c_int32), ('t1', ctypes.c_int32), ('resume', ctypes.c_int32), ('phase', ctypes.c_int32)]
 
 
+class SSRtmArgs(ctypes.Structure):
+    """Mirror of ss_rtm_args (one gradient / images call of the native driver)."""
+    _fields_ = [('stages', ctypes.c_int32), ('segment', ctypes.c_int32), ('impl', ctypes.c_int32),
+                ('obs_repeat', ctypes.c_int32), ('laplacian', ctypes.c_int32), ('timing', ctypes.c_int32),
+                ('dt', _d), ('dx', _d), ('t_mute', _d), ('ramp', _d), ('eps', _d), ('k', _d),
+                ('c2', _vp), ('A', _vp), ('B', _vp), ('w', _vp), ('amp', _vp), ('sd', _vp), ('v', _vp),
+                ('obs', _vp), ('v_mute', _vp), ('seed', _vp), ('gathers', _vp), ('grad', _vp), ('illum', _vp),
+                ('img', _vp)]
+
+
 SWEEP_MAX_GROUPS = 4
 _F = ctypes.c_float
 # FP32-only entry points of the learned preconditioner
@@ -82,6 +93,12 @@ _SWEEP = {
     'ss_sweep_autotune_f32': [_i, ctypes.POINTER(SSSweepGroup), _i, _i, _vp],
 }
 
+_RTM = {
+    'ss_rtm_f32': [ctypes.POINTER(SSGeom), ctypes.POINTER(SSRtmArgs), _vp, _i64, _vp],
+    'ss_rtm_f64': [ctypes.POINTER(SSGeom), ctypes.POINTER(SSRtmArgs), _vp, _i64, _vp],
+    'ss_rtm_timing': [ctypes.POINTER(_i), ctypes.POINTER(_d), ctypes.POINTER(_i), ctypes.POINTER(_d)],
+}
+
 _QC = {
     'ss_first_breaks_f32': [_i64, _i, _i, _vp, _d, _vp, _vp, _vp],
     'ss_first_breaks_f64': [_i64, _i, _i, _vp, _d, _vp, _vp, _vp],
@@ -90,7 +107,7 @@ _QC = {
 }
 
 EXPORTS = ['ss_last_error', 'ss_version', 'ss_resident_max_cells', 'ss_workspace_bytes'] + [
-    f'ss_{name}_{sfx}' for sfx in ('f32', 'f64') for name in _sig(ctypes.c_float)] + list(_CNN) + list(_SWEEP) + list(_QC)
+    f'ss_{name}_{sfx}' for sfx in ('f32', 'f64') for name in _sig(ctypes.c_float)] + list(_CNN) + list(_SWEEP) + list(_QC) + list(_RTM)
 
 _LIB = None
 
@@ -123,7 +140,7 @@ def load():
             fn = getattr(lib, f'ss_{name}_{sfx}')
             fn.argtypes = args
             fn.restype = ctypes.c_int
-    for name, args in list(_CNN.items()) + list(_SWEEP.items()) + list(_QC.items()):
+    for name, args in list(_CNN.items()) + list(_SWEEP.items()) + list(_QC.items()) + list(_RTM.items()):
         fn_ = getattr(lib, name)
         fn_.argtypes = args
         fn_.restype = ctypes.c_int
@@ -139,6 +156,15 @@ def fn(name, dtype):
     if sfx is None:
         raise TypeError(f'shotsweep kernels support float32 and float64, not {dtype}')
     return getattr(load(), f'ss_{name}_{sfx}')
+
+
+def rtm_timing():
+    """{'forward': (launches, ms), 'adjoint': (launches, ms)} of this thread's last
+    ss_rtm_* call made with timing = 1 (waits for those launches)."""
+    nf, na = ctypes.c_int(), ctypes.c_int()
+    mf, ma = ctypes.c_double(), ctypes.c_double()
+    check(load().ss_rtm_timing(ctypes.byref(nf), ctypes.byref(mf), ctypes.byref(na), ctypes.byref(ma)), 'rtm timing')
+    return {'forward': (nf.value, mf.value), 'adjoint': (na.value, ma.value)}
 
 
 def check(rc, what=''):

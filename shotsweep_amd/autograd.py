@@ -2,9 +2,10 @@
 
 ``modeling(v, sim)`` returns the gathers (B, ns, nt, nx) of velocity models
 v (B, nz, nx); its backward is the hand-written exact discrete adjoint
-(``Sim._adjoint_tape``), seeded with the incoming gradient, recomputing the
-forward from checkpoints every ``sim.checkpoint_interval`` steps (or reading
-the stored Laplacian history when the Sim has ``history='full'``).  For
+(``Sim._adjoint_tape``, one native ss_rtm call), seeded with the incoming
+gradient, recomputing the forward from checkpoints every
+``sim.checkpoint_interval`` steps (or reading the stored Laplacian history
+when the Sim has ``history='full'``).  For
 ``0.5 * ((modeling(v) - obs)**2).sum()`` this is the reference's misfit
 gradient (openfwi/physics/gpu.py:152-194, summed over shots).
 
@@ -18,8 +19,7 @@ import torch
 class _Modeling(torch.autograd.Function):
     @staticmethod
     def forward(ctx, v, sim):
-        v64 = v.detach().to('cpu', torch.float64).numpy()
-        prep = sim._prepare(v64)
+        prep = sim._prepare(v.detach())   # coefficients formed on the device, no host round trip
         keep = v.requires_grad
         # the tape: checkpoints every checkpoint_interval steps with recompute,
         # unless the Sim was built with history='full'

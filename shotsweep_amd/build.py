@@ -37,17 +37,34 @@ def sources():
     return sorted(CSRC.glob('*.hip'))
 
 
-def build(verbose=False, extra=()):
-    """Build the shared library; returns its path.  Always recompiles."""
-    tmp = LIB.with_suffix('.so.tmp')
-    cmd = [hipcc(), *FLAGS, *extra, *map(str, sources()), '-o', str(tmp)]
+def _compile(cmd, verbose):
     if verbose:
         print(' '.join(cmd), flush=True)
-    proc = subprocess.run(cmd, capture_output=True, text=True)
-    if proc.returncode != 0:
-        raise RuntimeError(f'hipcc failed ({proc.returncode}):\n{proc.stderr[-8000:]}')
-    if verbose and proc.stderr.strip():
-        print(proc.stderr[-4000:], file=sys.stderr)
+    return subprocess.run(cmd, capture_output=True, text=True)
+
+
+def build(verbose=False, extra=(), jobs=None):
+    """Build the shared library; returns its path.  Always recompiles: every
+    source is compiled to an object in parallel (each keeps its own device code;
+    no kernel is launched across translation units), then linked."""
+    from concurrent.futures import ThreadPoolExecutor
+    import tempfile
+    tmp = LIB.with_suffix('.so.tmp')
+    cflags = [f for f in FLAGS if f != '-shared']
+    with tempfile.TemporaryDirectory(prefix='shotsweep_build_') as d:
+        objs = [Path(d) / (src.stem + '.o') for src in sources()]
+        cmds = [[hipcc(), *cflags, *extra, '-c', str(src), '-o', str(obj)] for src, obj in zip(sources(), objs)]
+        workers = jobs or min(len(cmds), max(1, min(8, os.cpu_count() or 1)))
+        with ThreadPoolExecutor(workers) as pool:
+            procs = list(pool.map(lambda c: _compile(c, verbose), cmds))
+        for proc in procs:
+            if proc.returncode != 0:
+                raise RuntimeError(f'hipcc failed ({proc.returncode}):\n{proc.stderr[-8000:]}')
+            if verbose and proc.stderr.strip():
+                print(proc.stderr[-4000:], file=sys.stderr)
+        proc = _compile([hipcc(), *FLAGS, *extra, *map(str, objs), '-o', str(tmp)], verbose)
+        if proc.returncode != 0:
+            raise RuntimeError(f'hipcc link failed ({proc.returncode}):\n{proc.stderr[-8000:]}')
     os.replace(tmp, LIB)
     return LIB
 

@@ -289,27 +289,12 @@ extern "C" {
 
 const char *ss_last_error(void) { return ss::g_last_error.c_str(); }
 
-int ss_version(void) { return 5; }
+int ss_version(void) { return 6; }
 
 int64_t ss_resident_max_cells(void) { return resident_max_cells(); }
 
 int64_t ss_workspace_bytes(const ss_geom *gp, int segment, int elem_size) {
-  ss::Geom g;
-  if (ss::check_geom(gp, &g) != SS_OK) return -1;
-  if (segment < 1 || (elem_size != 4 && elem_size != 8)) {
-    ss::set_error("workspace_bytes: segment must be positive and elem_size 4 or 8");
-    return -1;
-  }
-  const int64_t plane = (int64_t)g.ns * g.H * g.W * elem_size;  // one state plane per velocity row
-  const int64_t traces = (int64_t)g.ns * g.nt * g.nx * elem_size;
-  int64_t per_row;
-  if (segment >= g.nt) {  // whole history; p0, p1, m1, m2 and the imaging sum
-    per_row = plane * g.nt + 5 * plane + 2 * traces;
-  } else {                // checkpoints and two alternating segment buffers with their own p0, p1
-    const int64_t nck = (g.nt + segment - 1) / segment;
-    per_row = plane * 2 * nck + 2 * plane * segment + 9 * plane + 2 * traces;
-  }
-  return per_row * g.nb;
+  return rtm_workspace_bytes(gp, segment, elem_size);
 }
 
 int ss_forward_f32(const ss_geom *g, int t0, int t1, const float *c2, const float *A, const float *B,

@@ -60,11 +60,14 @@ __host__ __device__ __forceinline__ T lap_sum(T zm2, T zm1, T c, T zp1, T zp2, T
 // -60 q(c), then -q, +16q, +16q, -q along z (destination shifts -2,-1,+1,+2),
 // then the same along x.  Gathered, destination shift d reads q at c - d.
 // q is zero outside the interior, so absent taps contribute an exact +-0.
-// The first term is 0 - 60 q, as the reference subtracts from a zero buffer.
+// The reference's first term is 0 - 60 q (it subtracts from a zero buffer);
+// -60 q is the same number for every q except q = +0, where it is -0 instead
+// of +0.  A zero's sign never changes a nonzero sum or product, so every
+// nonzero value downstream is the same bit for bit, and zeros compare equal.
 template <typename T>
 __host__ __device__ __forceinline__ T lapt_sum(T qc, T qzp2, T qzp1, T qzm1, T qzm2, T qxp2, T qxp1, T qxm1,
                                                T qxm2) {
-  T s = T(0) - T(60) * qc;
+  T s = T(-60) * qc;
   s = s - qzp2;
   s = fma_exact(T(16), qzp1, s);
   s = fma_exact(T(16), qzm1, s);

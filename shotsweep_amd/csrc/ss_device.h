@@ -62,6 +62,20 @@ __device__ __forceinline__ float2 lds_table2(uint32_t byte_off) {
 }
 // 8-byte read at an LDS byte address (same addressing as the tables).
 __device__ __forceinline__ float2 lds_ld2(uint32_t byte_addr) { return lds_table2(byte_addr); }
+// 8-byte write and 16-byte read at LDS byte addresses.
+typedef float ss_v4f __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(3))) ss_v2f ss_lds_v2f;
+typedef __attribute__((address_space(3))) const ss_v4f ss_lds_cv4f;
+__device__ __forceinline__ void lds_st2(uint32_t byte_addr, float2 v) {
+  ss_v2f w;
+  w.x = v.x;
+  w.y = v.y;
+  *reinterpret_cast<ss_lds_v2f *>((uintptr_t)byte_addr) = w;
+}
+__device__ __forceinline__ float4 lds_ld4(uint32_t byte_addr) {
+  const ss_v4f v = *reinterpret_cast<ss_lds_cv4f *>((uintptr_t)byte_addr);
+  return make_float4(v.x, v.y, v.z, v.w);
+}
 __device__ __forceinline__ float lds_table_a(uint32_t byte_off) {
   return *reinterpret_cast<ss_lds_cfloat *>((uintptr_t)byte_off);
 }

@@ -65,12 +65,23 @@ int velocity_scores(int B, int H, int W, const T *pred, const T *truth, double v
                     const double *window, double *out, hipStream_t st);
 size_t scores_lds_bytes(int H, int W);
 
+// Native gradient / images driver (ss_rtm.hip).
+int64_t rtm_workspace_bytes(const ss_geom *g, int segment, int elem_size);
+
 // Mixed-size tiled sweep (FP32); dir 0 = forward, 1 = adjoint.
 int sweep_run(int ngroups, const ss_sweep_group *groups, int dir, int use_graph, hipStream_t st);
 int sweep_autotune(int ngroups, ss_sweep_group *groups, int dir, int max_trials, hipStream_t st);
+bool sweep_tuned(int dir, int H, int W, int pad);           // autotuned tiling recorded for this grid
+int sweep_launches(const ss_sweep_group &group, int dir);  // launches of one call (< 0: error)
+bool sweep_fits(const Geom &g);                             // within the tiled kernels' limits
 
+// c2a: NULL, or the plane c2 * A of c2a_plane (used by the 150 x 150 paired kernel,
+// see resident_uses_c2a; other shapes ignore it).
 int adjoint_resident(const Geom &g, int t0, int t1, const float *c2, const float *A, const float *B, const float *w,
                      const float *sd, const float *res, const float *hist, float *m1, float *m2, float *gacc,
-                     float *illum, float *src_grad, hipStream_t st);
+                     float *illum, float *src_grad, hipStream_t st, const float *c2a = nullptr);
+bool resident_uses_c2a(const Geom &g);
+// out (nb, H, W) = c2 * A on the interior cells, 0 on the two-cell rim
+int c2a_plane(int64_t nb, int H, int W, const float *c2, const float *A, float *out, hipStream_t st);
 
 }  // namespace ss

@@ -88,21 +88,45 @@ __device__ __forceinline__ uint32_t cell_desc(int i, int H, int W, int nz, int n
   return v;
 }
 
-// Cell descriptor of the paired kernels (16 bits): bit 0 Laplacian interior,
-// bit 1 model cell, and in bits 3-13 the byte offset of the cell's 8-byte
-// sponge-table entry, (A, B) for the forward and (2A, A B) for the adjoint.
-// Model cells sit at sponge depth 0, i.e. entry 0 (A = B = 1).  The table
-// starts at LDS address 0, so a lookup is one mask and the load itself.
-constexpr uint32_t kPInterior = 1u, kPModel = 2u, kPTabMask = 0x3ff8u;
-template <int SHIFT>
-__device__ __forceinline__ uint32_t pair_cell_desc(int i, int H, int W, int nz, int nx, int pad) {
-  const int z = i / W, x = i - z * W;
-  uint32_t v = (uint32_t)(sponge_dist(z, H, pad) * (pad + 1) + sponge_dist(x, W, pad)) << SHIFT;
-  if (z >= pad && z < pad + nz && x >= pad && x < pad + nx) v |= kPModel;
+// Pair descriptor of the paired kernels (32 bits, one per cell pair (2q, 2q+1)):
+//   bit 0       Laplacian interior (both cells share it: the two-cell rim has even width)
+//   bits 4-14   byte offset of the pair's 16-byte sponge entry in the pair table,
+//               which starts at LDS address 0 (a lookup is one mask and one read)
+//   bits 16-30  model pairs: byte offset of the pair's illumination in the model plane
+//   bit 31      model pair (both cells are model cells: the model starts on an even
+//               column and has even width), so one signed compare tests it
+// With pad and the model width even, a pair starts on an even column and the
+// sponge depth of its first cell, dx0, fixes the second's: 0 in the model
+// columns, dx0 - 1 in the left sponge (dx0 even), dx0 + 1 in the right one
+// (dx0 odd).  So the pair table has one entry per (dz, dx0), dz, dx0 in
+// [0, pad], holding both cells' coefficients: (A0, B0, A1, B1) for the forward,
+// (2A0, A0 B0, 2A1, A1 B1) for the adjoint (2A is exact and A = 0.5 * 2A).
+// Model pairs sit at sponge depth 0, entry 0 (A = B = 1).
+constexpr uint32_t kPInterior = 1u, kPTabMask = 0x7ff0u, kPIlMask = 0x7fffu;
+__device__ __forceinline__ uint32_t pair_desc(int q, int H, int W, int nz, int nx, int pad) {
+  const int c = 2 * q, z = c / W, x = c - z * W;
+  uint32_t v = (uint32_t)(sponge_dist(z, H, pad) * (pad + 1) + sponge_dist(x, W, pad)) << 4;
   if (z >= 2 && z < H - 2 && x >= 2 && x < W - 2) v |= kPInterior;
+  if (z >= pad && z < pad + nz && x >= pad && x < pad + nx)
+    v |= 0x80000000u | ((uint32_t)(((z - pad) * nx + (x - pad)) * 4) << 16);
   return v;
 }
-__device__ __forceinline__ float2 sponge_at(uint32_t d) { return lds_table2(d & kPTabMask); }
+__device__ __forceinline__ float4 pair_coef(uint32_t d) { return lds_ld4(d & kPTabMask); }
+
+// Pair table entry (dz, dx0) from the full A, B planes at a representative pair
+// of that depth in the top sponge (row pad - dz): columns pad (dx0 = 0),
+// pad - dx0 (dx0 even, left) or W - 1 - pad + dx0 (dx0 odd, right).
+// mode 0: (A0, B0, A1, B1); mode 1: (2A0, A0 B0, 2A1, A1 B1).
+__device__ void fill_pair_table(float4 *tab, const float *A, const float *B, int W, int pad, int mode) {
+  const int n = (pad + 1) * (pad + 1);
+  for (int k = threadIdx.x; k < n; k += blockDim.x) {
+    const int dz = k / (pad + 1), dx0 = k - dz * (pad + 1);
+    const int x0 = dx0 == 0 ? pad : ((dx0 & 1) ? W - 1 - pad + dx0 : pad - dx0);
+    const int c = (pad - dz) * W + x0;
+    const float a0 = A[c], b0 = B[c], a1 = A[c + 1], b1 = B[c + 1];
+    tab[k] = mode ? make_float4(2.0f * a0, a0 * b0, 2.0f * a1, a1 * b1) : make_float4(a0, b0, a1, b1);
+  }
+}
 
 // Per-cell descriptors are loop-invariant; left visible, the compiler hoists
 // every value derived from them (table addresses, flag masks) out of the time
@@ -134,19 +158,6 @@ __device__ void fill_table(float2 *tab, const float *A, const float *B, int W, i
   }
 }
 
-// Adjoint table (dz, dx) -> (2A, A*B) in 8-byte entries; A itself is 0.5 * 2A, exact (A is a
-// normal number in (0, 1]), so one 8-byte read gives all three coefficients.
-__device__ void fill_table2a(float2 *tab, const float *A, const float *B, int W, int pad) {
-  const int n = (pad + 1) * (pad + 1);
-  for (int k = threadIdx.x; k < n; k += blockDim.x) {
-    const int dz = k / (pad + 1), dx = k - dz * (pad + 1);
-    const int c = (pad - dz) * W + (pad - dx);
-    const float a = A[c], b = B[c];
-    tab[k] = make_float2(2.0f * a, a * b);
-  }
-}
-__device__ __forceinline__ float2 sponge2a_at(uint32_t d) { return lds_table2(d & kPTabMask); }
-__device__ __forceinline__ float sponge2a_a(uint32_t d) { return 0.5f * lds_table_a(d & kPTabMask); }
 
 // Workgroup -> propagation.  Workgroups are dealt round-robin to the 8 XCDs
 // (blockIdx % 8), so XCD x receives blocks x, x+8, ...; mapping those to the
@@ -176,14 +187,17 @@ size_t adj_lds_bytes(const Geom &g) {
   return fwd_lds_bytes(g) + (size_t)g.nz * g.nx * sizeof(float);
 }
 
-// Paired kernels (even plane width): [sponge table | guard | plane | guard | illumination (adjoint)].
+// Paired kernels (even plane width): [pair table | illumination (adjoint) | guard | plane | guard].
+// The illumination sits right after the table, so its byte offsets (bits 16-30 of a
+// model pair's descriptor) are relative to an address the LDS instruction adds as
+// an immediate.
 constexpr int pair_guard(int W) { return 2 * W + 2; }
+constexpr int pair_table_floats(int pad) { return (pad + 1) * (pad + 1) * 4; }
 size_t fwd_pair_lds_bytes(const Geom &g) {
-  return (size_t)(g.pad + 1) * (g.pad + 1) * sizeof(float2) + (size_t)(g.H * g.W + 2 * pair_guard(g.W)) * sizeof(float);
+  return (size_t)(pair_table_floats(g.pad) + g.H * g.W + 2 * pair_guard(g.W)) * sizeof(float);
 }
 size_t adj_pair_lds_bytes(const Geom &g) {
-  return (size_t)(g.pad + 1) * (g.pad + 1) * sizeof(float2) +
-         (size_t)(g.H * g.W + 2 * pair_guard(g.W) + g.nz * g.nx) * sizeof(float);
+  return (size_t)(pair_table_floats(g.pad) + g.nz * g.nx + g.H * g.W + 2 * pair_guard(g.W)) * sizeof(float);
 }
 
 // NZ, NX, PAD > 0 and ROW >= 0 fix the geometry at compile time (0 / -1: runtime).
@@ -313,10 +327,11 @@ __global__ __launch_bounds__(kFwdThreads) void fwd_resident_kernel(Geom g, int t
 // Paired forward for planes of even width: thread t owns the cell pairs
 // (2q, 2q+1), q = t + 1024 j, so its nine-tap stencils come from seven aligned
 // 8-byte LDS reads per pair (the row taps c-2..c+3 and the pairs two and one
-// rows above and below), and each Laplacian store moves two cells.  Per cell
-// the arithmetic is that of fwd_resident_kernel; every flag the forward needs
+// rows above and below), one 16-byte read gives both cells' sponge
+// coefficients, and each Laplacian store moves two cells.  Per cell the
+// arithmetic is that of fwd_resident_kernel; every flag the forward needs
 // (interior, model, receiver row) starts on an even column, so a pair never
-// mixes the cases except in its two sponge-table indices.
+// mixes the cases.
 template <int KP, int NZ, int NX, int PAD, int ROW>
 __global__ __launch_bounds__(kFwdThreads) void fwd_pair_kernel(Geom g, int t0, int t1, const float *__restrict__ c2,
                                                                const float *__restrict__ A,
@@ -330,10 +345,10 @@ __global__ __launch_bounds__(kFwdThreads) void fwd_pair_kernel(Geom g, int t0, i
   constexpr int nz = NZ, nx = NX, pad = PAD, row = ROW;
   constexpr int H = nz + 2 * pad, W = nx + 2 * pad;
   constexpr int HW = H * W, NP = HW / 2;
-  static_assert(W % 2 == 0 && HW % 2 == 0, "pairs need an even plane width");
-  static_assert((pad + 1) * (pad + 1) * 8 <= (int)kPTabMask + 8, "sponge-table offsets must fit the descriptor");
-  constexpr int guard = pair_guard(W), tabf = (pad + 1) * (pad + 1) * 2;
-  float2 *tab = reinterpret_cast<float2 *>(lds);  // at LDS address 0: lookups need no base add
+  static_assert(W % 2 == 0 && nx % 2 == 0 && pad % 2 == 0, "pairs need even widths");
+  static_assert((pad + 1) * (pad + 1) * 16 <= (int)kPTabMask + 16, "pair-table offsets must fit the descriptor");
+  constexpr int guard = pair_guard(W), tabf = pair_table_floats(pad);
+  float4 *tab = reinterpret_cast<float4 *>(lds);  // at LDS address 0: lookups need no base add
   float *P = lds + tabf + guard;
   float2 *P2 = reinterpret_cast<float2 *>(P);
   constexpr uint32_t kPAddr = (uint32_t)(tabf + guard) * 4;  // LDS byte address of P
@@ -346,12 +361,11 @@ __global__ __launch_bounds__(kFwdThreads) void fwd_pair_kernel(Geom g, int t0, i
     P[k - guard] = 0.0f;
     P[HW + k] = 0.0f;
   }
-  fill_table(tab, A, B, W, pad, 0);
+  fill_pair_table(tab, A, B, W, pad, 0);
 
   float2 pn[KP], cc[KP];
-  uint32_t pk[KP];  // two 16-bit cell descriptors per pair
-  // Laplacian-interior flag of a pair: the 2-cell rim has even width, so both
-  // cells of a pair share it.  Kept as a lane mask, it costs one select per cell.
+  uint32_t pk[KP];  // pair descriptors
+  // Laplacian-interior flag of a pair, kept as a lane mask: one select per cell.
   bool inner[KP];
   const float2 *p0p = reinterpret_cast<const float2 *>(p0g + prop * HW);
   const float2 *p1p = reinterpret_cast<const float2 *>(p1g + prop * HW);
@@ -366,7 +380,7 @@ __global__ __launch_bounds__(kFwdThreads) void fwd_pair_kernel(Geom g, int t0, i
       pn[j] = p0p[q];
       P2[q] = p1p[q];
       cc[j] = c2b[q];
-      pk[j] = pair_cell_desc<3>(2 * q, H, W, nz, nx, pad) | (pair_cell_desc<3>(2 * q + 1, H, W, nz, nx, pad) << 16);
+      pk[j] = pair_desc(q, H, W, nz, nx, pad);
       inner[j] = (pk[j] & kPInterior) != 0;
     }
   }
@@ -376,6 +390,7 @@ __global__ __launch_bounds__(kFwdThreads) void fwd_pair_kernel(Geom g, int t0, i
   const float ampv = amp[prop];
   const int nslots = t1 - t0;
   const int nck = ckpt ? (nslots + ckpt_every - 1) / ckpt_every : 0;
+  // receivers: thread t records columns t, t + 1024, ... of the receiver row
   const bool rec_thread = gathers && t < nx;
   // Without a history buffer the resource has size 0 and the hardware drops
   // the Laplacian stores, so the step loop needs no branch for them.
@@ -390,7 +405,7 @@ __global__ __launch_bounds__(kFwdThreads) void fwd_pair_kernel(Geom g, int t0, i
     const int slot = it - t0;
     opaque(pk);
     if (rec_thread && it > t0)  // p(it) after injection is the sample of step it-1
-      buf_store(P[rec0 + t], grs, t * 4, (it - 1) * nx * 4);
+      for (int x = t; x < nx; x += kFwdThreads) buf_store(P[rec0 + x], grs, x * 4, (it - 1) * nx * 4);
     if (ckpt && slot % ckpt_every == 0) {
       const int base = (slot / ckpt_every) * 2 * HW;
 #pragma unroll
@@ -407,7 +422,6 @@ __global__ __launch_bounds__(kFwdThreads) void fwd_pair_kernel(Geom g, int t0, i
     for (int j = 0; j < KP; ++j) {
       asm volatile("" : "+v"(pk[j]));  // one pair at a time
       if (t + j * kFwdThreads < NP) {
-        const uint32_t d0 = pk[j] & 0xffffu, d1 = pk[j] >> 16;
         // One base address per pair (two rows up); the seven taps are
         // immediate offsets from it.
         uint32_t a = kPAddr + 8u * (uint32_t)(off + j * kFwdThreads - W);
@@ -419,9 +433,9 @@ __global__ __launch_bounds__(kFwdThreads) void fwd_pair_kernel(Geom g, int t0, i
         const float s1 = lap_sum(u2m.y, um.y, r1.y, up.y, u2p.y, r0.y, r1.x, r2.x, r2.y);
         const float2 Lq = div12x2(s0, s1);
         const float L0 = inner[j] ? Lq.x : 0.0f, L1 = inner[j] ? Lq.y : 0.0f;
-        const float2 ab0 = sponge_at(d0), ab1 = sponge_at(d1);
-        pn[j].x = leapfrog(r1.x, pn[j].x, ab0.y, cc[j].x, L0, ab0.x);
-        pn[j].y = leapfrog(r1.y, pn[j].y, ab1.y, cc[j].y, L1, ab1.x);
+        const float4 ab = pair_coef(pk[j]);  // (A0, B0, A1, B1)
+        pn[j].x = leapfrog(r1.x, pn[j].x, ab.y, cc[j].x, L0, ab.x);
+        pn[j].y = leapfrog(r1.y, pn[j].y, ab.w, cc[j].y, L1, ab.z);
         buf_store2(make_float2(L0, L1), hrs, off * 8, (slot * HW + 2 * j * kFwdThreads) * 4);
         asm volatile("" : "+v"(pn[j].x), "+v"(pn[j].y));  // finish this pair before the next starts
       }
@@ -439,7 +453,8 @@ __global__ __launch_bounds__(kFwdThreads) void fwd_pair_kernel(Geom g, int t0, i
     if (src_owner) P[srci] = P[srci] + w[it] * ampv;  // gpu.py:114, added after damping
     __syncthreads();
   }
-  if (rec_thread) buf_store(P[rec0 + t], grs, t * 4, (t1 - 1) * nx * 4);
+  if (rec_thread)
+    for (int x = t; x < nx; x += kFwdThreads) buf_store(P[rec0 + x], grs, x * 4, (t1 - 1) * nx * 4);
   float2 *p0o = reinterpret_cast<float2 *>(p0g + prop * HW);
   float2 *p1o = reinterpret_cast<float2 *>(p1g + prop * HW);
 #pragma unroll
@@ -610,15 +625,22 @@ __global__ __launch_bounds__(kAdjThreads) void adj_resident_kernel(Geom g, int t
 
 // Paired adjoint for planes of even width (the 150 x 150 plane): thread t owns
 // the cell pairs (2q, 2q+1), q = t + 512 j.  Seven aligned 8-byte LDS reads per
-// pair give both cells' lap_adjoint taps; the Laplacian history, residual and
-// c2 move two cells per buffer instruction; the illumination of a model pair
-// is one 8-byte read-modify-write.  Per cell the arithmetic and its order are
-// those of adj_resident_kernel.  Interior, model and receiver-row ranges all
-// start on even columns, so both cells of a pair share those flags.
+// pair give both cells' lap_adjoint taps and one 16-byte read their sponge
+// coefficients; the Laplacian history, residual and c2 move two cells per
+// buffer instruction; the illumination of a model pair is one 8-byte
+// read-modify-write at the address its descriptor holds.  Per cell the
+// arithmetic and its order are those of adj_resident_kernel.  Interior, model
+// and receiver-row ranges all start on even columns, so both cells of a pair
+// share those flags.
+//
+// C2A = true: `c2` holds the plane c2 * A (rounded once, zero on the two-cell
+// rim, made by c2a_kernel), the first product of q = ((c2 A) m1) / 12, so the
+// q update needs neither the sponge table nor the interior flag.  C2A = false:
+// `c2` is c2 itself and A comes from the table.
 // history loads are issued this many pairs ahead (a divisor of the pair count)
 template <int KP>
 constexpr int pair_prefetch() { return KP % 2 == 0 ? 2 : (KP % 3 == 0 ? 3 : 1); }
-template <int KP, int NZ, int NX, int PAD, int ROW>
+template <int KP, int NZ, int NX, int PAD, int ROW, bool C2A>
 __global__ __launch_bounds__(kAdjThreads) void adj_pair_kernel(Geom g, int t0, int t1, const float *__restrict__ c2,
                                                                const float *__restrict__ A,
                                                                const float *__restrict__ B,
@@ -634,35 +656,32 @@ __global__ __launch_bounds__(kAdjThreads) void adj_pair_kernel(Geom g, int t0, i
   constexpr int H = nz + 2 * pad, W = nx + 2 * pad;
   constexpr int HW = H * W, NP = HW / 2;
   static_assert(W % 2 == 0 && nx % 2 == 0 && pad % 2 == 0, "pairs need even widths");
-  static_assert((pad + 1) * (pad + 1) * 8 <= (int)kPTabMask + 8, "sponge-table offsets must fit the descriptor");
-  constexpr int guard = pair_guard(W), tabf = (pad + 1) * (pad + 1) * 2;
-  float2 *tab = reinterpret_cast<float2 *>(lds);  // at LDS address 0: lookups need no base add
-  float *Q = lds + tabf + guard;
+  static_assert((pad + 1) * (pad + 1) * 16 <= (int)kPTabMask + 16, "pair-table offsets must fit the descriptor");
+  static_assert(nz * nx * 4 <= (int)kPIlMask + 1, "illumination offsets must fit the descriptor");
+  constexpr int nzx = nz * nx;
+  constexpr int guard = pair_guard(W), tabf = pair_table_floats(pad);
+  float4 *tab = reinterpret_cast<float4 *>(lds);  // at LDS address 0: lookups need no base add
+  float *il = lds + tabf;
+  constexpr uint32_t kIlAddr = (uint32_t)tabf * 4;  // LDS byte address of the illumination
+  float *Q = il + nzx + guard;
   float2 *Q2 = reinterpret_cast<float2 *>(Q);
-  constexpr uint32_t kQAddr = (uint32_t)(tabf + guard) * 4;  // LDS byte address of Q
-  float *il = Q + HW + guard;
-  float2 *il2 = reinterpret_cast<float2 *>(il);
+  constexpr uint32_t kQAddr = (uint32_t)(tabf + nzx + guard) * 4;  // LDS byte address of Q
   const int64_t prop = xcd_prop(g);
   if (prop < 0) return;
   const int b = (int)(prop / g.ns), s = (int)(prop - (int64_t)b * g.ns);
   const int t = threadIdx.x;
-  constexpr int nzx = nz * nx;
   for (int k = t; k < guard; k += kAdjThreads) {
     Q[k - guard] = 0.0f;
     Q[HW + k] = 0.0f;
   }
-  fill_table2a(tab, A, B, W, pad);
+  fill_pair_table(tab, A, B, W, pad, 1);
   for (int k = t; k < nzx; k += kAdjThreads) il[k] = illumg[prop * nzx + k];
   __syncthreads();
   uint32_t pk[KP];
 #pragma unroll
   for (int j = 0; j < KP; ++j) {
     const int q = t + j * kAdjThreads;
-    // bit 31 (the second cell's unused bit 15) repeats the model flag, so one
-    // signed compare tests it
-    const uint32_t d0 = pair_cell_desc<3>(2 * q, H, W, nz, nx, pad);
-    pk[j] = q < NP ? (d0 | (pair_cell_desc<3>(2 * q + 1, H, W, nz, nx, pad) << 16) | ((d0 & kPModel) ? 0x80000000u : 0u))
-                   : 0u;
+    pk[j] = q < NP ? pair_desc(q, H, W, nz, nx, pad) : 0u;
   }
   const __amdgpu_buffer_rsrc_t srs1 = make_rsrc(m1g + prop * HW, HW);
   const __amdgpu_buffer_rsrc_t srs2 = make_rsrc(m2g + prop * HW, HW);
@@ -682,10 +701,14 @@ __global__ __launch_bounds__(kAdjThreads) void adj_pair_kernel(Geom g, int t0, i
   for (int j = 0; j < KP; ++j) {
     const int q = t + j * kAdjThreads;
     if (q < NP) {
-      const uint32_t d0 = pk[j] & 0xffffu, d1 = pk[j] >> 16;
       const float2 cv = buf_load2(crs, t * 8, j * kAdjThreads * 8);
-      Q2[q] = make_float2((d0 & kPInterior) ? div12((cv.x * sponge2a_a(d0)) * m1[j].x) : 0.0f,
-                          (d1 & kPInterior) ? div12((cv.y * sponge2a_a(d1)) * m1[j].y) : 0.0f);
+      if (C2A) {
+        Q2[q] = div12x2(cv.x * m1[j].x, cv.y * m1[j].y);
+      } else {
+        const float4 e = pair_coef(pk[j]);
+        const float2 qq = div12x2((cv.x * (0.5f * e.x)) * m1[j].x, (cv.y * (0.5f * e.z)) * m1[j].y);
+        Q2[q] = make_float2(keep_if<0>(qq.x, pk[j]), keep_if<0>(qq.y, pk[j]));
+      }
     }
   }
   constexpr int rec0 = (pad + row) * W + pad, rec1 = rec0 + nx;
@@ -700,7 +723,7 @@ __global__ __launch_bounds__(kAdjThreads) void adj_pair_kernel(Geom g, int t0, i
   __syncthreads();
   constexpr int D = pair_prefetch<KP>();
   static_assert(KP % D == 0, "pair count must be a multiple of the prefetch distance");
-  float2 Lb[D], Cb[D];  // history and c2 of the pairs D ahead
+  float2 Lb[D], Cb[D];  // history and c2 (or c2 A) of the pairs D ahead
 #pragma unroll
   for (int d = 0; d < D; ++d) {
     const bool live = t + d * kAdjThreads < NP;
@@ -727,7 +750,6 @@ __global__ __launch_bounds__(kAdjThreads) void adj_pair_kernel(Geom g, int t0, i
       }
       if (t + j * kAdjThreads < NP) {
         const int q = (int)(o8 >> 3) + j * kAdjThreads;
-        const uint32_t d0 = pk[j] & 0xffffu, d1 = pk[j] >> 16;
         // One base address per pair (two rows up); the seven taps are
         // immediate offsets from it.
         uint32_t a = o8 + (kQAddr + 8u * (uint32_t)(j * kAdjThreads - W));
@@ -737,13 +759,13 @@ __global__ __launch_bounds__(kAdjThreads) void adj_pair_kernel(Geom g, int t0, i
         const float2 u2p = lds_ld2(a + 16 * W);
         const float lt0 = lapt_sum(r1.x, u2p.x, up.x, um.x, u2m.x, r2.x, r1.y, r0.y, r0.x);
         const float lt1 = lapt_sum(r1.y, u2p.y, up.y, um.y, u2m.y, r2.y, r2.x, r1.x, r0.y);
-        const float2 e0 = sponge2a_at(d0), e1 = sponge2a_at(d1);  // (2A, A B)
-        float mu0 = e0.x * m1[j].x;
-        float mu1 = e1.x * m1[j].y;
+        const float4 e = pair_coef(pk[j]);  // (2A0, A0 B0, 2A1, A1 B1)
+        float mu0 = e.x * m1[j].x;
+        float mu1 = e.z * m1[j].y;
         mu0 = mu0 + lt0;
         mu1 = mu1 + lt1;
-        mu0 = mu0 - e0.y * m2[j].x;
-        mu1 = mu1 - e1.y * m2[j].y;
+        mu0 = mu0 - e.y * m2[j].x;
+        mu1 = mu1 - e.w * m2[j].y;
         if (j * kAdjThreads < rec1 / 2 && (j + 1) * kAdjThreads > rec0 / 2) {
           if (q >= rec0 / 2 && q < rec1 / 2) {
             const float2 rv = buf_load2(rrs, (2 * q - rec0) * 4, it * nx * 4);
@@ -751,16 +773,17 @@ __global__ __launch_bounds__(kAdjThreads) void adj_pair_kernel(Geom g, int t0, i
             mu1 = mu1 + rv.y;
           }
         }
-        ga[j].x = ga[j].x + ((0.5f * e0.x) * mu0) * L.x;  // A = 0.5 (2A), exact
-        ga[j].y = ga[j].y + ((0.5f * e1.x) * mu1) * L.y;
+        ga[j].x = ga[j].x + ((0.5f * e.x) * mu0) * L.x;  // A = 0.5 (2A), exact
+        ga[j].y = ga[j].y + ((0.5f * e.z) * mu1) * L.y;
         if (j == srcq / kAdjThreads) {
           if (q == srcq) sg = sg + (((srci & 1) ? mu1 : mu0) * wt) * sdv;
         }
-        if ((int32_t)pk[j] < 0) {  // model pair
-          const int c = 2 * q, z = c / W, x = c - z * W;
-          float2 &v = il2[((z - pad) * nx + (x - pad)) >> 1];
+        if ((int32_t)pk[j] < 0) {  // model pair: its illumination address is in the descriptor
+          const uint32_t ia = kIlAddr + ((pk[j] >> 16) & kPIlMask);
+          float2 v = lds_ld2(ia);
           v.x = v.x + L.x * L.x;
           v.y = v.y + L.y * L.y;
+          lds_st2(ia, v);
         }
         m2[j] = m1[j];
         m1[j] = make_float2(mu0, mu1);
@@ -782,10 +805,14 @@ __global__ __launch_bounds__(kAdjThreads) void adj_pair_kernel(Geom g, int t0, i
         if (t + jn * kAdjThreads < NP) Cb[j % D] = buf_load2(crs, off * 8, jn * kAdjThreads * 8);
       }
       if (t + j * kAdjThreads < NP) {
-        const uint32_t d0 = pk[j] & 0xffffu, d1 = pk[j] >> 16;
-        const float2 qq = div12x2((cv.x * sponge2a_a(d0)) * m1[j].x, (cv.y * sponge2a_a(d1)) * m1[j].y);
-        // both cells of a pair share the interior flag (bit 0)
-        Q2[off + j * kAdjThreads] = make_float2(keep_if<0>(qq.x, pk[j]), keep_if<0>(qq.y, pk[j]));
+        if (C2A) {
+          Q2[off + j * kAdjThreads] = div12x2(cv.x * m1[j].x, cv.y * m1[j].y);
+        } else {
+          const float4 e = pair_coef(pk[j]);
+          const float2 qq = div12x2((cv.x * (0.5f * e.x)) * m1[j].x, (cv.y * (0.5f * e.z)) * m1[j].y);
+          // both cells of a pair share the interior flag (bit 0)
+          Q2[off + j * kAdjThreads] = make_float2(keep_if<0>(qq.x, pk[j]), keep_if<0>(qq.y, pk[j]));
+        }
       }
     }
     __syncthreads();
@@ -800,6 +827,19 @@ __global__ __launch_bounds__(kAdjThreads) void adj_pair_kernel(Geom g, int t0, i
   }
   if (own_src) src_grad[prop] = sg;
   for (int k = t; k < nzx; k += kAdjThreads) illumg[prop * nzx + k] = il[k];
+}
+
+// c2a = c2 * A on the interior cells of every plane of c2 (nb, H, W), zero on the
+// two-cell rim: the first product of the adjoint's q = ((c2 A) m1) / 12 (gpu.py:179),
+// rounded once as there.  Zero on the rim makes q zero there, as lap_adjoint only
+// reads p[2:-2, 2:-2] (gpu.py:41).
+__global__ void c2a_kernel(int64_t n, int H, int W, const float *__restrict__ c2, const float *__restrict__ A,
+                           float *__restrict__ out) {
+  const int64_t HW = (int64_t)H * W;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n * HW; i += (int64_t)gridDim.x * blockDim.x) {
+    const int c = (int)(i % HW), z = c / W, x = c - z * W;
+    out[i] = (z >= 2 && z < H - 2 && x >= 2 && x < W - 2) ? c2[i] * A[c] : 0.0f;
+  }
 }
 
 int device_lds_limit() {
@@ -882,20 +922,32 @@ int forward_resident(const Geom &g, int t0, int t1, const float *c2, const float
 
 int adjoint_resident(const Geom &g, int t0, int t1, const float *c2, const float *A, const float *B, const float *w,
                      const float *sd, const float *res, const float *hist, float *m1, float *m2, float *gacc,
-                     float *illum, float *src_grad, hipStream_t st) {
+                     float *illum, float *src_grad, hipStream_t st, const float *c2a) {
   const size_t lds = is_openfwi(g) ? adj_pair_lds_bytes(g) : adj_lds_bytes(g);
   return dispatch_adj(g, [&](auto kc, auto fixed) -> int {
     constexpr bool F = decltype(fixed)::value;
-    // the fixed 150 x 150 plane runs the paired kernel (22 pairs per thread)
-    auto kern = F ? adj_pair_kernel<(150 * 150 / 2 + kAdjThreads - 1) / kAdjThreads, 70, 70, 40, 1>
+    // the fixed 150 x 150 plane runs the paired kernel (22 pairs per thread), with the
+    // c2 * A plane when the caller has one
+    constexpr int KP = (150 * 150 / 2 + kAdjThreads - 1) / kAdjThreads;
+    const bool pre = F && c2a != nullptr;
+    auto kern = F ? (pre ? adj_pair_kernel<KP, 70, 70, 40, 1, true> : adj_pair_kernel<KP, 70, 70, 40, 1, false>)
                   : adj_resident_kernel<decltype(kc)::value, F ? 70 : 0, F ? 70 : 0, F ? 40 : 0, F ? 1 : -1>;
     hipError_t e = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return hip_status(e, "adjoint_resident attribute");
     if (F && !no_static_lds((const void *)kern)) return hip_status(hipErrorInvalidValue, "adjoint_resident static LDS");
-    kern<<<dim3((unsigned)xcd_blocks((int64_t)g.nb * g.ns)), dim3(kAdjThreads), lds, st>>>(g, t0, t1, c2, A, B, w, sd, res,
-                                                                                 hist, m1, m2, gacc, illum, src_grad);
+    kern<<<dim3((unsigned)xcd_blocks((int64_t)g.nb * g.ns)), dim3(kAdjThreads), lds, st>>>(
+        g, t0, t1, pre ? c2a : c2, A, B, w, sd, res, hist, m1, m2, gacc, illum, src_grad);
     return hip_status(hipGetLastError(), "adjoint_resident launch");
   });
+}
+
+bool resident_uses_c2a(const Geom &g) { return is_openfwi(g); }
+
+int c2a_plane(int64_t nb, int H, int W, const float *c2, const float *A, float *out, hipStream_t st) {
+  const int64_t n = nb * H * W;
+  const int blocks = (int)std::min<int64_t>((n + 255) / 256, 4096);
+  c2a_kernel<<<dim3(blocks), dim3(256), 0, st>>>(nb, H, W, c2, A, out);
+  return hip_status(hipGetLastError(), "c2a launch");
 }
 
 }  // namespace ss

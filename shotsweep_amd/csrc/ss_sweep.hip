@@ -29,6 +29,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <deque>
 #include <map>
 #include <mutex>
 #include <string>
@@ -983,17 +984,53 @@ void launch_range(const Plan &p, KernelFn fn, int k0, int k1, hipStream_t st) {
 // kMaxGraphs.  A checkpointed gradient replays two graphs per segment (forward
 // recompute and adjoint, each with its own step range), so the cache holds the
 // 40 layouts of 1000 steps in segments of 50 several times over.
+//
+// Every launch of a cached graph records the graph's own event on the
+// launching stream.  An evicted graph may still be queued there, so it is not
+// destroyed at once: it moves to a retired list with its event and is
+// destroyed by a later call once that event has completed (hipEventQuery), so
+// eviction never waits for the device or for other streams.  Only when more
+// than kMaxRetired graphs are pending does a call wait, and then only for the
+// oldest retired graph's own event.
 constexpr size_t kMaxGraphs = 256;
+constexpr size_t kMaxRetired = 64;
 struct CachedGraph {
   hipGraphExec_t exec;
+  hipEvent_t done;     // recorded after each launch of exec
   uint64_t last_use;
 };
 struct GraphCache {
   std::map<std::string, CachedGraph> exec;
   std::map<int, hipStream_t> capture_stream;
+  std::deque<CachedGraph> retired;
   uint64_t clock = 0;
 };
 GraphCache g_graphs;
+
+void destroy_graph(const CachedGraph &c) {
+  (void)hipGraphExecDestroy(c.exec);
+  (void)hipEventDestroy(c.done);
+}
+
+// Destroy the retired graphs whose last launch has completed (caller holds g_mutex).
+int reap_retired(bool wait_oldest) {
+  if (wait_oldest && !g_graphs.retired.empty()) {
+    hipError_t e = hipEventSynchronize(g_graphs.retired.front().done);
+    if (e != hipSuccess) return hip_status(e, "sweep graph retire");
+  }
+  for (auto it = g_graphs.retired.begin(); it != g_graphs.retired.end();) {
+    hipError_t q = hipEventQuery(it->done);
+    if (q == hipSuccess) {
+      destroy_graph(*it);
+      it = g_graphs.retired.erase(it);
+    } else if (q == hipErrorNotReady) {
+      ++it;
+    } else {
+      return hip_status(q, "sweep graph retire query");
+    }
+  }
+  return SS_OK;
+}
 
 std::string graph_key(const Plan &p, int dev) {
   std::string key(reinterpret_cast<const char *>(&p.args), sizeof p.args);
@@ -1016,6 +1053,7 @@ int run_plan(const Plan &p, int use_graph, hipStream_t st) {
   std::lock_guard<std::mutex> lock(g_mutex);
   const std::string key = graph_key(p, dev);
   auto it = g_graphs.exec.find(key);
+  rc = SS_OK;
   hipGraphExec_t exec = nullptr;
   if (it != g_graphs.exec.end()) {
     exec = it->second.exec;
@@ -1034,12 +1072,11 @@ int run_plan(const Plan &p, int use_graph, hipStream_t st) {
       auto old = g_graphs.exec.begin();
       for (auto i = g_graphs.exec.begin(); i != g_graphs.exec.end(); ++i)
         if (i->second.last_use < old->second.last_use) old = i;
-      // the evicted graph may still be queued on some stream: let it finish first
-      e = hipDeviceSynchronize();
-      if (e != hipSuccess) return hip_status(e, "sweep graph eviction");
-      (void)hipGraphExecDestroy(old->second.exec);
+      g_graphs.retired.push_back(old->second);  // destroyed once its last launch has completed
       g_graphs.exec.erase(old);
     }
+    rc = reap_retired(g_graphs.retired.size() > kMaxRetired);
+    if (rc != SS_OK) return rc;
     hipGraph_t graph = nullptr;
     e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
     if (e != hipSuccess) return hip_status(e, "sweep begin capture");
@@ -1054,9 +1091,17 @@ int run_plan(const Plan &p, int use_graph, hipStream_t st) {
     e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
     if (e != hipSuccess) return hip_status(e, "sweep graph instantiate");
-    g_graphs.exec[key] = CachedGraph{exec, ++g_graphs.clock};
+    hipEvent_t done;
+    e = hipEventCreateWithFlags(&done, hipEventDisableTiming);
+    if (e != hipSuccess) {
+      (void)hipGraphExecDestroy(exec);
+      return hip_status(e, "sweep graph event");
+    }
+    it = g_graphs.exec.emplace(key, CachedGraph{exec, done, ++g_graphs.clock}).first;
   }
-  return hip_status(hipGraphLaunch(exec, st), "sweep graph launch");
+  e = hipGraphLaunch(exec, st);
+  if (e != hipSuccess) return hip_status(e, "sweep graph launch");
+  return hip_status(hipEventRecord(it->second.done, st), "sweep graph event record");
 }
 
 // Kernel variant and LDS floors a group's items get in a joint launch: the
@@ -1110,6 +1155,28 @@ int sweep_run(int ngroups, const ss_sweep_group *groups, int dir, int use_graph,
   int rc = make_plan(ngroups, groups, dir, &p);
   if (rc != SS_OK) return rc;
   return run_plan(p, use_graph, st);
+}
+
+bool sweep_tuned(int dir, int H, int W, int pad) {
+  std::lock_guard<std::mutex> lock(g_mutex);
+  return g_tuned.count(tune_key(dir, H, W, pad)) != 0;
+}
+
+int sweep_launches(const ss_sweep_group &group, int dir) {
+  SGroup o;
+  char err[256] = {0};
+  const int rc = fill_group(group, dir, &o, err, sizeof err);
+  if (rc != SS_OK) {
+    if (err[0]) set_error(err);
+    return rc;
+  }
+  return o.nlaunch;
+}
+
+bool sweep_fits(const Geom &g) {
+  const int64_t HW = (int64_t)g.H * g.W;
+  return HW <= kMaxPlane && (g.pad + 1) * (g.pad + 1) <= kMaxTable && g.nt * HW <= (int64_t)0x1fffffff &&
+         (int64_t)g.nt * g.nx <= (int64_t)0x1fffffff;
 }
 
 int sweep_autotune(int ngroups, ss_sweep_group *groups, int dir, int max_trials, hipStream_t st) {

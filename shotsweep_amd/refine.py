@@ -51,7 +51,7 @@ class Problem:
     def misfit(self, v):
         """Per-model data misfit 0.5*sum((F(v) - obs)^2), v a (B, nz, nx) device tensor."""
         sim = self.sim
-        gathers = sim.forward(v.detach().double().cpu().numpy())[0]
+        gathers = sim.forward(v.detach())[0]
         out = torch.empty(self.B, dtype=torch.float32, device=sim.device)
         _lib.check(_lib.load().ss_sumsq_diff_f32(self.B, self.ntr, _lib.ptr(gathers), _lib.ptr(self.obs),
                                                  _lib.ptr(out), sim._stream()), 'sumsq_diff')
@@ -61,7 +61,7 @@ class Problem:
         """CNN input (B, 2, nz, nx) and per-model misfit at v (one forward + adjoint)."""
         sim = self.sim
         g = sim.geom
-        prep = sim._prepare(v.detach().double().cpu().numpy())
+        prep = sim._prepare(v.detach())
         grad, illum, gathers = sim._gradient_device(prep, self.obs, 1, None)
         x = torch.empty(self.B, 2, g.nz, g.nx, dtype=torch.float32, device=sim.device)
         gscale = torch.empty(self.B, dtype=torch.float32, device=sim.device)

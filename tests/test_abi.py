@@ -96,18 +96,35 @@ def test_sweep_validation_without_gpu_work():
 
 
 def test_workspace_bytes_closed_form():
-    """ss_workspace_bytes (host arithmetic): whole history = history planes + 5
-    state planes + 2 trace sets per row; checkpoints every s steps = 2 ceil(nt/s)
-    checkpoint planes + 2 segment buffers of s planes + 9 state planes + 2 trace
-    sets.  The reference's own estimate (gpu.py:70-73) is a different scheme."""
+    """ss_workspace_bytes (host arithmetic) is the sum of the native call's workspace
+    regions, each starting on a 256-byte boundary: per propagation two trace sets
+    (gathers, residual), the source term, the imaging sum and 8 state planes; then
+    the whole history (nt planes) or 2 ceil(nt/s) checkpoint planes plus two
+    segment buffers of s planes; FP32 adds the c2*A plane per row and, for
+    segments, the tiled path's coefficient / illumination copies.  The
+    reference's own estimate (gpu.py:70-73) is a different scheme."""
+    def size(nb, ns, nz, nx, pad, nt, seg, e):
+        H, W = nz + 2 * pad, nx + 2 * pad
+        nprop, HW = nb * ns, H * W
+        regions = ([nb * HW * 4] if e == 4 else []) + [nprop * nt * nx * e] * 2 + [nprop * e, nprop * HW * e,
+                                                                                  8 * nprop * HW * e]
+        if seg >= nt:
+            regions.append(nprop * nt * HW * e)
+        else:
+            nck = -(-nt // seg)
+            regions += [nprop * nck * 2 * HW * e, nprop * seg * HW * e, nprop * seg * HW * e]
+            if e == 4:
+                regions += [nb * HW * 4, nprop * 4, nprop * 4, nprop * nz * nx * 4, nprop * 4]
+        return sum(-(-r // 256) * 256 for r in regions)
+
     g = _lib.make_geom(192, 5, 70, 70, 40, 1000, 1, (0, 17, 34, 52, 69))
-    plane, traces = 5 * 150 * 150 * 4, 5 * 1000 * 70 * 4
-    assert _lib.workspace_bytes(g, 1000, 4) == 192 * (1000 * plane + 5 * plane + 2 * traces)
+    for seg, e in ((1000, 4), (50, 4), (73, 8), (1000, 8)):
+        assert _lib.workspace_bytes(g, seg, e) == size(192, 5, 70, 70, 40, 1000, seg, e), (seg, e)
     assert _lib.workspace_bytes(g, 5000, 4) == _lib.workspace_bytes(g, 1000, 4)
-    assert _lib.workspace_bytes(g, 50, 4) == 192 * (40 * plane + 100 * plane + 9 * plane + 2 * traces)
-    assert _lib.workspace_bytes(g, 73, 8) == 2 * 192 * (2 * 14 * plane + 2 * 73 * plane + 9 * plane + 2 * traces)
+    g2 = _lib.make_geom(3, 7, 11, 13, 5, 37, 1, tuple(range(7)))
+    assert _lib.workspace_bytes(g2, 10, 4) == size(3, 7, 11, 13, 5, 37, 10, 4)
     # 86.4 GB of Laplacian history for the 960 shot-images of the batched workload
-    assert abs(_lib.workspace_bytes(g, 1000, 4) / 1e9 - 87.0) < 0.5
+    assert abs(_lib.workspace_bytes(g, 1000, 4) / 1e9 - 87.7) < 0.2
     with pytest.raises(ValueError, match='segment must be positive'):
         _lib.workspace_bytes(g, 0, 4)
     with pytest.raises(ValueError, match='geometry'):
