@@ -176,6 +176,21 @@ typedef struct ss_rtm_args {
 int ss_rtm_f32(const ss_geom *g, const ss_rtm_args *a, void *workspace, int64_t workspace_bytes, void *stream);
 int ss_rtm_f64(const ss_geom *g, const ss_rtm_args *a, void *workspace, int64_t workspace_bytes, void *stream);
 
+/*
+ * ss_rtm_f32 without a workspace argument and without any device allocation:
+ * no counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The workspace is a fixed 256 MiB device array inside the
+ * library (one copy per GPU, held until the process ends); a call whose
+ * ss_workspace_bytes(g, a->segment, 4) exceeds it fails with SS_EINVAL.  Every
+ * launch is a direct launch on `stream` (no side stream, no graphs, no
+ * autotuner trials, no timing events: a->timing is ignored).  All calls on one
+ * GPU share the array and it keeps the last call's contents: calls that
+ * overlap on one GPU race, and a forward-only call's tape survives only until
+ * the next call on that GPU.
+ */
+int ss_rtm_noalloc_f32(const ss_geom *g, const ss_rtm_args *a, void *stream);
+
 /* Launch counts and summed durations (ms) of the forward and adjoint time-loop
  * launches of this thread's last ss_rtm_* call with timing = 1 (waits for them). */
 int ss_rtm_timing(int *nforward, double *forward_ms, int *nadjoint, double *adjoint_ms);

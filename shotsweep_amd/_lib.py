@@ -96,6 +96,7 @@ _SWEEP = {
 _RTM = {
     'ss_rtm_f32': [ctypes.POINTER(SSGeom), ctypes.POINTER(SSRtmArgs), _vp, _i64, _vp],
     'ss_rtm_f64': [ctypes.POINTER(SSGeom), ctypes.POINTER(SSRtmArgs), _vp, _i64, _vp],
+    'ss_rtm_noalloc_f32': [ctypes.POINTER(SSGeom), ctypes.POINTER(SSRtmArgs), _vp],
     'ss_rtm_timing': [ctypes.POINTER(_i), ctypes.POINTER(_d), ctypes.POINTER(_i), ctypes.POINTER(_d)],
 }
 

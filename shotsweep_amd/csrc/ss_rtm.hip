@@ -313,19 +313,23 @@ ss_sweep_group tiled_group(const ss_geom &gg, const Layout<float> &L, const floa
   return q;
 }
 
-int tiled_run(const ss_sweep_group &q, int dir, hipStream_t st, int *launches) {
+int tiled_run(const ss_sweep_group &q, int dir, hipStream_t st, int *launches, bool use_graph) {
   const int n = sweep_launches(q, dir);
   if (n < 0) return n;
   *launches = n;
   size_t m = 0;
   SS_TRY(mark(dir, st, &m));
-  SS_TRY(sweep_run(1, &q, dir, /*use_graph=*/1, st));
+  SS_TRY(sweep_run(1, &q, dir, use_graph ? 1 : 0, st));
   return mark_end(m, st);
 }
 
 // --------------------------------------------------------------- the driver
+// noalloc: run everything on `st` with direct launches: no side stream, no events,
+// no captured graphs and no autotuner trials (untuned grids take the tiled kernels'
+// cost-model tiling; any tiling gives the same results).
 template <typename T>
-int rtm(const ss_geom *gg, const ss_rtm_args *a, void *ws, int64_t ws_bytes, hipStream_t st, bool own_streams) {
+int rtm(const ss_geom *gg, const ss_rtm_args *a, void *ws, int64_t ws_bytes, hipStream_t st, bool noalloc) {
+  const bool own_streams = !noalloc, use_graph = !noalloc;
   Geom g;
   SS_TRY(check_geom(gg, &g));
   if (!a) {
@@ -394,7 +398,7 @@ int rtm(const ss_geom *gg, const ss_rtm_args *a, void *ws, int64_t ws_bytes, hip
         SS_TRY(copy_rows<float>(L.tamp, 0, amp, 0, 1, nprop, st));
         float *fst = L.state;  // 2 buffers x (p0, p1)
         // the autotuner's trial launches overwrite the state: tune before the first segment
-        if (!sweep_tuned(0, g.H, g.W, g.pad)) {
+        if (!noalloc && !sweep_tuned(0, g.H, g.W, g.pad)) {
           ss_sweep_group q = tiled_group(*gg, L, A, B, w, 0, 0, segment, 0, 0, fst, L.seg[0]);
           SS_TRY(sweep_autotune(1, &q, 0, 0, st));
         }
@@ -406,7 +410,7 @@ int rtm(const ss_geom *gg, const ss_rtm_args *a, void *ws, int64_t ws_bytes, hip
             SS_TRY(copy_rows<float>(L.ckpt + (int64_t)(2 * k + pl) * HW, (int64_t)nck * 2 * HW,
                                     k ? fst + (2 * phase + pl) * nprop * HW : nullptr, HW, nprop, HW, st));
           int n = 0;
-          SS_TRY(tiled_run(tiled_group(*gg, L, A, B, w, 0, t0, t1, k > 0, phase, fst, nullptr), 0, st, &n));
+          SS_TRY(tiled_run(tiled_group(*gg, L, A, B, w, 0, t0, t1, k > 0, phase, fst, nullptr), 0, st, &n, use_graph));
           phase = (phase + n) & 1;
         }
       }
@@ -495,7 +499,7 @@ int rtm(const ss_geom *gg, const ss_rtm_args *a, void *ws, int64_t ws_bytes, hip
       SS_TRY(copy_rows<float>(L.tc2, 0, c2, 0, 1, g.nb * HW, st));
       SS_TRY(copy_rows<float>(L.tsd, 0, sd, 0, 1, nprop, st));
       float *fst = L.plane(0, nprop, HW), *ast = L.plane(4, nprop, HW);
-      if (!sweep_tuned(0, g.H, g.W, g.pad)) {
+      if (!noalloc && !sweep_tuned(0, g.H, g.W, g.pad)) {
         int t0, t1;
         span(nck - 1, &t0, &t1);
         ss_sweep_group q = tiled_group(*gg, L, A, B, w, 0, t0, t1, 0, 0, fst, L.seg[0]);
@@ -515,12 +519,12 @@ int rtm(const ss_geom *gg, const ss_rtm_args *a, void *ws, int64_t ws_bytes, hip
           SS_TRY(copy_rows<float>(fst + pl * nprop * HW, HW, L.ckpt + (int64_t)(2 * k + pl) * HW,
                                   (int64_t)nck * 2 * HW, nprop, HW, side));
         int n = 0;
-        SS_TRY(tiled_run(tiled_group(*gg, L, A, B, w, 0, t0, t1, 1, 0, fst, L.seg[i]), 0, side, &n));
+        SS_TRY(tiled_run(tiled_group(*gg, L, A, B, w, 0, t0, t1, 1, 0, fst, L.seg[i]), 0, side, &n, use_graph));
         if (overlap) SS_TRY(hip_status(hipEventRecord(R->ready[i], side), "ready"));
         return SS_OK;
       };
       SS_TRY(recompute(nck - 1));
-      if (!sweep_tuned(1, g.H, g.W, g.pad)) {
+      if (!noalloc && !sweep_tuned(1, g.H, g.W, g.pad)) {
         // the adjoint trials read the last segment's history; the first real adjoint
         // call (resume 0) then overwrites the outputs the trials left
         if (overlap) SS_TRY(hip_status(hipStreamWaitEvent(st, R->ready[(nck - 1) & 1], 0), "ready wait"));
@@ -537,7 +541,7 @@ int rtm(const ss_geom *gg, const ss_rtm_args *a, void *ws, int64_t ws_bytes, hip
         int t0, t1;
         span(k, &t0, &t1);
         int n = 0;
-        SS_TRY(tiled_run(tiled_group(*gg, L, A, B, w, 1, t0, t1, k < nck - 1, phase, ast, L.seg[i]), 1, st, &n));
+        SS_TRY(tiled_run(tiled_group(*gg, L, A, B, w, 1, t0, t1, k < nck - 1, phase, ast, L.seg[i]), 1, st, &n, use_graph));
         phase = (phase + n) & 1;
         if (overlap) {
           SS_TRY(hip_status(hipEventRecord(R->freed[i], st), "freed"));
@@ -566,8 +570,61 @@ int rtm_entry(const ss_geom *g, const ss_rtm_args *a, void *ws, int64_t ws_bytes
   t_timing.on = a && a->timing;
   t_timing.used = 0;
   t_timing.marks.clear();
-  const int rc = rtm<T>(g, a, ws, ws_bytes, (hipStream_t)stream, true);
-  return rc;
+  return rtm<T>(g, a, ws, ws_bytes, (hipStream_t)stream, false);
+}
+
+// ------------------------------------------------- no-allocation entry point
+// A fixed device array serves as the workspace of ss_rtm_noalloc_f32, for
+// callers that cannot allocate device memory (code under graph capture, tight
+// serving loops).  Each GPU the library runs on holds its own copy until the
+// process ends.  Its host-side name is no device address: the address of the
+// current device's copy comes from hipGetSymbolAddress, once per device, and
+// only that address reaches kernels.
+constexpr int64_t kScratchBytes = 256ll << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+std::mutex g_scratch_mutex;
+std::map<int, void *> g_scratch;  // device -> address of its copy of scratch
+
+int scratch_address(void **out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_status(e, "rtm_noalloc device");
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  auto it = g_scratch.find(dev);
+  if (it == g_scratch.end()) {
+    void *p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return hip_status(e, "rtm_noalloc scratch address");
+    if (!p) {
+      set_error("rtm_noalloc: no device address for the scratch array");
+      return SS_EHIP;
+    }
+    it = g_scratch.emplace(dev, p).first;
+  }
+  *out = it->second;
+  return SS_OK;
+}
+
+int rtm_noalloc(const ss_geom *g, const ss_rtm_args *a, void *stream) {
+  if (!g || !a) {
+    set_error("rtm_noalloc: geometry and args are required");
+    return SS_EINVAL;
+  }
+  const int64_t need = rtm_workspace_bytes(g, a->segment, 4);
+  if (need < 0) return SS_EINVAL;
+  if (need > kScratchBytes) {
+    set_error("rtm_noalloc: this call needs " + std::to_string(need) + " bytes of workspace; the fixed scratch "
+              "holds " + std::to_string(kScratchBytes) + " (256 MiB): use fewer rows or a shorter segment, or "
+              "ss_rtm_f32 with a workspace");
+    return SS_EINVAL;
+  }
+  void *ws = nullptr;
+  SS_TRY(scratch_address(&ws));
+  t_timing.on = false;
+  t_timing.used = 0;
+  t_timing.marks.clear();
+  return rtm<float>(g, a, ws, kScratchBytes, (hipStream_t)stream, true);
 }
 
 }  // namespace
@@ -627,6 +684,15 @@ int ss_rtm_f64(const ss_geom *g, const ss_rtm_args *a, void *workspace, int64_t 
     return ss::rtm_entry<double>(g, a, workspace, workspace_bytes, stream);
   } catch (const std::exception &e) {
     ss::set_error(std::string("rtm: ") + e.what());
+    return SS_EHIP;
+  }
+}
+
+int ss_rtm_noalloc_f32(const ss_geom *g, const ss_rtm_args *a, void *stream) {
+  try {
+    return ss::rtm_noalloc(g, a, stream);
+  } catch (const std::exception &e) {
+    ss::set_error(std::string("rtm_noalloc: ") + e.what());
     return SS_EHIP;
   }
 }
