@@ -1,22 +1,36 @@
 """Benchmark: batched forward + exact-adjoint RTM images on one MI355X.
 
-    python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--models 64] [--seed 0] [--no-cpu]
-    python bench.py --workload mixed [--models-per-size 32]    (uneven batch, see run_mixed)
-    python bench.py --workload autograd                        (differentiation, see run_autograd)
+    python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--models 64] [--seed 0] [--no-cpu] [--legs all|none|...]
+    python bench.py --workload mixed|autograd|refine   (one leg alone, its own JSON line)
 
-Workload (one "step"): 64 seeded 70 x 70 models of mixed structure (curved
-layers, a planar fault, smoothed texture) x 3 backgrounds (surface velocity +
-0.6/1.0/1.4 x depth trend) x 5 shots = 960 shot-images; nt = 1000, dt = 1 ms,
-dx = 10 m, pad 40 (150 x 150 padded grid), 15 Hz Ricker, FP32.  One step runs
-the forward with the Laplacian history, the muted residual, the exact adjoint
-with the imaging condition, the gradient fold and the image epilogue, on
-inputs resident in HBM (observed gathers, coefficient planes, mute velocities).
-Model generation, observed-data generation and host coefficient setup are
-outside the timed region.
+Headline workload (one "step"): 64 seeded 70 x 70 models of mixed structure
+(curved layers, a planar fault, smoothed texture) x 3 backgrounds (surface
+velocity + 0.6/1.0/1.4 x depth trend) x 5 shots = 960 shot-images; nt = 1000,
+dt = 1 ms, dx = 10 m, pad 40 (150 x 150 padded grid), 15 Hz Ricker, FP32.  One
+step is one native call (ss_rtm_f32): the forward with the Laplacian history,
+the muted residual, the exact adjoint with the imaging condition, the gradient
+fold and the image epilogue, on inputs resident in HBM.  `value` is the median
+over the timed steps (each bracketed by a synchronize).  The `protocol` object
+times the same call with the observed gathers copied from pinned host memory
+and the images and illumination copied back inside the timed region (median of
+three runs).  Model generation, observed-data generation and the host
+surface-velocity fits are outside every timed region.
 
-Prints one JSON line with the throughput, the roofline of the dominant kernel
-(HIP events on the launching stream) and the host-CPU baseline (the eager
-PyTorch oracle, timed on a bounded sample in this process).
+The roofline of the dominant kernel: its average launch time comes from HIP
+events recorded on the launching stream during the timed steps; its VALU
+instruction count, HBM traffic and LDS figures come from the rocprofv3 counter
+passes of the same workload (profiles/rtm_batched_counters.json).  The kernels
+keep the wavefield on chip, so the binding ceiling is instruction issue (one
+64-lane VALU instruction per clock per CU: 256 CUs x 2.4 GHz), reported as
+`frac`; the measured HBM traffic and the 16-byte-per-cell-update algorithmic
+rate sit beside it.
+
+The `legs` object holds the other configurations, each with its own host-CPU
+baseline (the oracle) and its error against the oracle: `differentiation`
+(16 models, checkpoints every 50 steps, autograd), `uneven` (32 models at each
+of 70x70 / 100x140 / 200x200 with their step and shot counts, one mixed sweep)
+and `refine` (the learned preconditioner against the plain step on a bounded
+sample).
 """
 import argparse
 import json
@@ -27,47 +41,65 @@ import time
 import numpy as np
 import torch
 
+ROOT = os.path.dirname(os.path.abspath(__file__))
 METRIC = 'RTM shot-images/s, 70x70 nt=1000 fwd+adjoint; learned vs plain refine misfit'
 PEAK_HBM_GBS = 8000.0
+CUS, PEAK_CLOCK_GHZ = 256, 2.4
+PEAK_VALU_TOPS = CUS * 64 * PEAK_CLOCK_GHZ / 1e3      # 64-lane VALU instructions: 39.3 T lane-ops/s
 BYTES_PER_CELL_UPDATE = 16          # read p(t-1), p(t), c2; write p(t+1); FP32
 
 
-def measured_traffic(kernel):
-    """HBM bytes per launch of a kernel of the batched workload, from the two
-    rocprofv3 counter passes (FETCH_SIZE, WRITE_SIZE) scaled by a 1 GiB copy,
-    recorded in profiles/hbm_traffic.json; None when not recorded."""
-    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'profiles', 'hbm_traffic.json')
+def log(msg):
+    print(f'[bench] {msg}', file=sys.stderr, flush=True)
+
+
+def counters():
+    """Per-kernel counter figures of the batched workload (rocprofv3 passes), or None."""
     try:
-        with open(path) as f:
-            return json.load(f)['kernels'][kernel]['bytes']
-    except (OSError, KeyError, ValueError):
+        with open(os.path.join(ROOT, 'profiles', 'rtm_batched_counters.json')) as f:
+            return json.load(f)
+    except (OSError, ValueError):
         return None
 
 
-def measured_lds(kernel, kernel_ms):
-    """LDS bytes per launch of a kernel of the batched workload (rocprofv3
-    SQ_INSTS_LDS_LOAD_BANDWIDTH + SQ_INSTS_LDS_STORE_BANDWIDTH, 64-byte units)
-    and its issue figures, from profiles/lds_valu.json, as an LDS roofline
-    beside the HBM one; None when not recorded."""
-    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'profiles', 'lds_valu.json')
-    try:
-        with open(path) as f:
-            rec = json.load(f)
-        k = rec['kernels'][kernel]
-        achieved = k['lds_bytes'] / (kernel_ms * 1e-3) / 1e9
-        return {'bytes': k['lds_bytes'], 'achieved': achieved, 'peak': rec['peak_lds_GBps'], 'unit': 'GB/s',
-                'frac': achieved / rec['peak_lds_GBps'], 'valu_busy': k['valu_busy'], 'lds_util': k['lds_util'],
-                'source': 'profiles/lds_valu.json (rocprofv3 LDS bandwidth and issue counters)'}
-    except (OSError, KeyError, ValueError):
-        return None
+def roofline(kernel, kernel_ms, alg_bytes):
+    """Roofline of one launch of `kernel` (HIP-event average kernel_ms) against its
+    binding ceiling, VALU issue; HBM and LDS fractions and the algorithmic rate beside it."""
+    t = kernel_ms * 1e-3
+    out = {'bound': 'valu', 'kernel': kernel, 'achieved': None, 'peak': PEAK_VALU_TOPS, 'unit': 'TOP/s',
+           'frac': None, 'traffic': None, 'kernel_ms': kernel_ms,
+           'algorithmic_bytes_per_launch': alg_bytes, 'algorithmic_rate_TBps': alg_bytes / t / 1e12,
+           'note': 'VALU lane-ops (64 per instruction) per second against 256 CUs x 64 lanes x 2.4 GHz; '
+                   'traffic = measured HBM bytes per launch (FETCH_SIZE x2 + WRITE_SIZE, scaled by a 1 GiB '
+                   'copy); the 16 B/cell-update algorithmic rate exceeds the HBM peak because the wavefield '
+                   'stays in LDS and registers'}
+    rec = counters()
+    k = rec and rec['kernels'].get(kernel)
+    if not k:
+        return out
+    achieved = k['valu_instructions'] * 64 / t / 1e12
+    out.update(achieved=achieved, frac=achieved / PEAK_VALU_TOPS, traffic=k['hbm_bytes'],
+               valu_instructions_per_launch=k['valu_instructions'], valu_per_pair_step=k['valu_per_pair_step'],
+               hbm={'achieved': k['hbm_bytes'] / t / 1e9, 'peak': PEAK_HBM_GBS, 'unit': 'GB/s',
+                    'frac': k['hbm_bytes'] / t / 1e9 / PEAK_HBM_GBS},
+               lds={'bytes': k['lds_bytes'], 'achieved': k['lds_bytes'] / t / 1e9, 'peak': rec['peak']['lds_GBps'],
+                    'unit': 'GB/s', 'frac': k['lds_bytes'] / t / 1e9 / rec['peak']['lds_GBps'],
+                    'bank_conflict_share': k['lds_bank_conflict_share']},
+               counter_run={'clock_GHz': k['clock_GHz'], 'valu_busy': k['valu_busy'], 'ms': k['counter_run_ms']},
+               source='profiles/rtm_batched_counters.json')
+    return out
 
 
+def median_rate(units, seconds):
+    return units / float(np.median(seconds))
+
+
+# ------------------------------------------------------------------ headline
 def workload(n_models, seed, sim):
     """Models, observed gathers (device), backgrounds and mute velocities."""
     from shotsweep_amd import background as BG, synth
     models = synth.mixed(n_models, seed=seed)
-    out, _, _ = sim.forward(models)
-    obs = out.contiguous()
+    obs = sim.forward(models)[0].contiguous()
     obs_host = obs.cpu().numpy()
     trend = BG.depth_trend(models)
     v_s, fallbacks = [], 0
@@ -116,6 +148,107 @@ def cpu_baseline(bgs, obs_host, v_mute, n_rows):
         np.concatenate(imgs), np.concatenate(illums), img64
 
 
+def protocol_runs(sim, bgs, obs_host, v_mute, runs=3):
+    """The images call with host buffers at both ends: observed gathers copied from
+    pinned host memory, coefficients formed from the host backgrounds, images and
+    illumination copied back to pinned host memory, all inside the timed region.
+    Returns the wall seconds of each run."""
+    g = sim.geom
+    obs_pin = torch.empty(obs_host.shape, dtype=torch.float32, pin_memory=True)
+    obs_pin.copy_(torch.from_numpy(obs_host))
+    obs_dev = torch.empty(obs_host.shape, dtype=torch.float32, device=sim.kdev)
+    nb = len(bgs)
+    img_pin = torch.empty(nb, g.ns, g.nz, g.nx, dtype=torch.float32, pin_memory=True)
+    ill_pin = torch.empty_like(img_pin)
+    vm_pin = torch.tensor(v_mute, dtype=torch.float32).pin_memory()
+    vm_dev = torch.empty(nb, dtype=torch.float32, device=sim.kdev)
+    out = []
+    for _ in range(runs + 1):                    # the first run warms the pinned paths
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        obs_dev.copy_(obs_pin, non_blocking=True)
+        vm_dev.copy_(vm_pin, non_blocking=True)
+        prep = sim._prepare(bgs)
+        img, illum = sim.images_device(prep, obs_dev, 3, vm_dev)
+        img_pin.copy_(img, non_blocking=True)
+        ill_pin.copy_(illum, non_blocking=True)
+        torch.cuda.synchronize()
+        out.append(time.perf_counter() - t0)
+    return out[1:], img_pin.numpy(), ill_pin.numpy()
+
+
+def run_batched(args):
+    from shotsweep_amd import gpu
+    dev = torch.device('cuda')
+    sim = gpu.Sim(device=dev, history=args.history)
+    models, obs, obs_host, bgs, v_mute, fallbacks = workload(args.models, args.seed, sim)
+    prep = sim._prepare(bgs)
+    vm = sim._tensor(v_mute).contiguous()
+    nb = prep['nb']
+    shot_images = nb * sim.geom.ns
+    seg = sim._segment(nb)
+    log(f'{args.models} models -> {nb} backgrounds -> {shot_images} shot-images; history segment {seg}; '
+        f'surface-velocity fallbacks {fallbacks}')
+    for _ in range(args.warmup):
+        sim.images_device(prep, obs, 3, vm)
+    torch.cuda.synchronize()
+    sim.timing, sim.timing_log = True, []
+    step_s = []
+    t_all = time.perf_counter()
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        img, illum = sim.images_device(prep, obs, 3, vm)
+        torch.cuda.synchronize()
+        step_s.append(time.perf_counter() - t0)
+    wall = time.perf_counter() - t_all
+    sim.timing = False
+    per_kernel = {}
+    for rec in sim.timing_log:
+        for name, (n, ms) in rec.items():
+            if n:
+                per_kernel.setdefault(name, []).append(ms / n)
+    avg = {k: float(np.mean(v)) for k, v in per_kernel.items()}
+    dominant = max(avg, key=avg.get)
+    H, W, nt = sim.H, sim.W, sim.geom.nt
+    alg_bytes = BYTES_PER_CELL_UPDATE * H * W * nt * shot_images       # one cell-update per cell-step per launch
+    ms_per_step = float(np.median(step_s)) * 1e3
+    log(f'{ms_per_step:.1f} ms/step (median of {args.steps}), kernels {avg}')
+    result = {
+        'metric': METRIC, 'value': shot_images / (ms_per_step * 1e-3), 'unit': 'shot-images/s', 'n_gpus': 1,
+        'steps': args.steps, 'warmup': args.warmup, 'ms_per_step': ms_per_step, 'higher_is_better': True,
+        'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f32', 'data': 'synthetic (seeded mixed-structure models)',
+        'config': {'workload': 'batched RTM: 64 mixed 70x70 models x 3 backgrounds x 5 shots = 960 shot-images, '
+                               'nt=1000, dt=1 ms, dx=10 m, pad 40 (150x150), 15 Hz Ricker',
+                   'models': args.models, 'shot_images': shot_images, 'seed': args.seed, 'history_segment': seg},
+        'timing': {'value_from': 'median of per-step wall times, inputs resident in HBM',
+                   'step_ms': [s * 1e3 for s in step_s], 'mean_rate': shot_images * args.steps / wall},
+        'roofline': roofline(dominant, avg[dominant], alg_bytes),
+        'kernel_ms': avg,
+    }
+    prot_s, pimg, pill = protocol_runs(sim, bgs, obs_host, v_mute)
+    result['protocol'] = {
+        'value': median_rate(shot_images, prot_s), 'unit': 'shot-images/s', 'runs_ms': [s * 1e3 for s in prot_s],
+        'timed': 'observed gathers H2D from pinned host memory + host coefficient setup + the native images call '
+                 '+ images and illumination D2H to pinned host memory; median of 3 runs after one warm-up',
+        'images_equal_device_resident': bool(np.array_equal(pimg, img.cpu().numpy()) and
+                                             np.array_equal(pill, illum.cpu().numpy()))}
+    log(f'protocol {result["protocol"]["value"]:.0f} shot-images/s')
+    if not args.no_cpu:
+        rows = 3 * args.cpu_models             # cpu_models models x 3 backgrounds x 5 shots
+        base, cimg, cillum, cimg64 = cpu_baseline(bgs, obs_host, v_mute, rows)
+        gimg = img[:rows].cpu().numpy()
+        err = float(np.linalg.norm(gimg - cimg) / np.linalg.norm(cimg))
+        result['cpu_baseline'] = base
+        result['speedup_vs_cpu'] = result['value'] / base['value']
+        result['image_rel_l2_vs_cpu_oracle'] = err
+        result['illum_bitwise_equal_cpu_oracle'] = bool(np.array_equal(illum[:rows].cpu().numpy(), cillum))
+        g3 = gimg[:3].astype(np.float64)
+        result['image_rel_l2_vs_fp64_oracle'] = float(np.linalg.norm(g3 - cimg64) / np.linalg.norm(cimg64))
+        log(f'cpu baseline {base["value"]:.1f} shot-images/s on {base["cores"]} threads')
+    return result
+
+
+# ---------------------------------------------------------------- uneven leg
 def mixed_workload(n_per_size, seed, acqs):
     """Per grid size: models, observed gathers (device), backgrounds and mute velocities.
 
@@ -158,108 +291,121 @@ def mixed_cpu_baseline(acqs, batches, shot_images):
                       f'{cores} threads; {", ".join(parts)}), extrapolated linearly to every row'}, imgs
 
 
-def run_mixed(args):
+def run_mixed(args, steps=None, warmup=None):
     """Uneven batch: 32 models each of 70x70 (nt 1000, 5 shots), 100x140 (nt 1500,
     7 shots) and 200x200 (nt 2000, 10 shots) x 3 backgrounds in one mixed sweep."""
     from shotsweep_amd.mixed import MixedSweep, config_acquisitions
+    steps = args.steps if steps is None else steps
+    warmup = args.warmup if warmup is None else warmup
     acqs = config_acquisitions()
     batches = mixed_workload(args.models_per_size, args.seed, acqs)
     ms = MixedSweep(acqs)
     shot_images = sum(len(b['v_bg']) * a.ns for a, b in zip(acqs, batches))
     rows = [len(b['v_bg']) for b in batches]
     nchunks, _ = ms._chunks(rows, [b['obs_repeat'] for b in batches])
-    print(f'[bench] mixed: rows {rows}, {shot_images} shot-images, {nchunks} chunks', file=sys.stderr, flush=True)
+    log(f'mixed: rows {rows}, {shot_images} shot-images, {nchunks} chunks')
     t0 = time.perf_counter()
-    for _ in range(args.warmup):
+    for _ in range(max(1, warmup)):
         ms.images_device(batches)
     torch.cuda.synchronize()
-    print(f'[bench] warmup (with autotuning) {time.perf_counter() - t0:.1f} s, tiles {ms.tiles}',
-          file=sys.stderr, flush=True)
+    log(f'mixed warmup (with autotuning) {time.perf_counter() - t0:.1f} s, tiles {ms.tiles}')
+    step_s = []
     ms.events = []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(steps):
+        t0 = time.perf_counter()
         out = ms.images_device(batches)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
+        torch.cuda.synchronize()
+        step_s.append(time.perf_counter() - t0)
     events, ms.events = ms.events, None
-    ms_per_step = wall / args.steps * 1e3
-    value = shot_images / (wall / args.steps)
     per = {}
     for name, a, b in events:
-        per[name] = per.get(name, 0.0) + a.elapsed_time(b) / args.steps
+        per[name] = per.get(name, 0.0) + a.elapsed_time(b) / steps
     dominant = max(per, key=per.get)
     cell_updates = sum(len(b['v_bg']) * a.ns * a.nt * (a.nz + 80) * (a.nx + 80) for a, b in zip(acqs, batches))
     alg_bytes = BYTES_PER_CELL_UPDATE * cell_updates
-    achieved = alg_bytes / (per[dominant] * 1e-3) / 1e9
-    print(f'[bench] {ms_per_step:.1f} ms/step, sweep ms per step {per}', file=sys.stderr, flush=True)
+    ms_per_step = float(np.median(step_s)) * 1e3
+    log(f'mixed {ms_per_step:.1f} ms/step, sweep ms per step {per}')
     result = {
-        'metric': METRIC, 'value': value, 'unit': 'shot-images/s', 'n_gpus': 1, 'steps': args.steps,
-        'warmup': args.warmup, 'ms_per_step': ms_per_step, 'higher_is_better': True, 'scaling': 'weak',
-        'vs_baseline': None, 'dtype': 'f32', 'data': 'synthetic (seeded mixed-structure models at each size)',
+        'metric': 'uneven-batch RTM shot-images/s', 'value': shot_images / (ms_per_step * 1e-3),
+        'unit': 'shot-images/s', 'steps': steps, 'warmup': max(1, warmup), 'ms_per_step': ms_per_step,
+        'step_ms': [s * 1e3 for s in step_s], 'dtype': 'f32',
         'config': {'workload': 'uneven batched RTM: 32 models each of 70x70 (nt 1000, 5 shots), 100x140 '
                                '(nt 1500, 7 shots), 200x200 (nt 2000, 10 shots) x 3 backgrounds, pad 40, '
                                'dx 10 m, 15 Hz Ricker', 'models_per_size': args.models_per_size,
                    'shot_images': shot_images, 'seed': args.seed, 'chunks': nchunks,
                    'tiles': {f'{a.nz}x{a.nx}': {('forward', 'adjoint')[d]: t for d, t in ms.tiles[k].items()}
                              for k, a in enumerate(acqs)}},
-        'roofline': {'bound': 'hbm', 'kernel': dominant, 'achieved': achieved, 'peak': PEAK_HBM_GBS, 'unit': 'GB/s',
-                     'frac': achieved / PEAK_HBM_GBS, 'traffic': None, 'sweep_ms_per_step': per,
-                     'algorithmic_bytes_per_direction': alg_bytes},
+        'roofline': {'bound': 'hbm', 'kernel': dominant, 'achieved': alg_bytes / (per[dominant] * 1e-3) / 1e9,
+                     'peak': PEAK_HBM_GBS, 'unit': 'GB/s', 'frac': None, 'traffic': None,
+                     'algorithmic_rate_frac': alg_bytes / (per[dominant] * 1e-3) / 1e9 / PEAK_HBM_GBS,
+                     'sweep_ms_per_step': per, 'algorithmic_bytes_per_direction': alg_bytes,
+                     'note': 'tiled kernels: state in LDS/registers within a launch, in HBM between launches; '
+                             'algorithmic 16 B per cell-update rate (measured traffic and issue in '
+                             'profiles/sweep_uneven_kernel_trace.md)'},
     }
     if not args.no_cpu:
         base, imgs = mixed_cpu_baseline(acqs, batches, shot_images)
         result['cpu_baseline'] = base
-        result['speedup_vs_cpu'] = value / base['value']
-        errs = []
+        result['speedup_vs_cpu'] = result['value'] / base['value']
+        errs, ill_eq = [], []
         for (cimg, cill), (gimg, gill) in zip(imgs, out):
             g = gimg[:1].cpu().numpy()
             errs.append(float(np.linalg.norm(g - cimg) / np.linalg.norm(cimg)))
-        result['image_rel_l2_vs_cpu_oracle'] = max(errs)
-    print(json.dumps(result), flush=True)
+            ill_eq.append(bool(np.array_equal(gill[:1].cpu().numpy(), cill)))
+        result['image_rel_l2_vs_cpu_oracle'] = errs
+        result['illum_bitwise_equal_cpu_oracle'] = ill_eq
+    return result
 
 
-def run_autograd(args):
+# -------------------------------------------------------- differentiation leg
+def run_autograd(args, steps=None, warmup=None):
     """Differentiation workload: misfit 0.5 * sum((F(v) - obs)^2) and its gradient through
     the autograd Function for 16 models x 5 shots (70 x 70, nt 1000), start = smoothed truth
     (sigma 8), checkpoints every 50 steps with recompute (the reference's scheme)."""
     from shotsweep_amd import autograd, gpu, synth
+    steps = args.steps if steps is None else steps
+    warmup = args.warmup if warmup is None else warmup
     sim = gpu.Sim(device='cuda', checkpoint_interval=50, history='checkpoint')
     true = synth.mixed(16, seed=args.seed)
     obs = sim.forward(true)[0].contiguous()
     start = synth.smooth(true, 8.0)
+    v = torch.tensor(start, dtype=torch.float32, device='cuda')
     ns = sim.geom.ns
-    for _ in range(args.warmup):
-        autograd.misfit_and_gradient(start, obs, sim)
+    for _ in range(max(1, warmup)):
+        autograd.misfit_and_gradient(v, obs, sim)
     torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        loss, grad = autograd.misfit_and_gradient(start, obs, sim)
-    torch.cuda.synchronize()
-    wall = (time.perf_counter() - t0) / args.steps
+    step_s = []
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        loss, grad = autograd.misfit_and_gradient(v, obs, sim)
+        torch.cuda.synchronize()
+        step_s.append(time.perf_counter() - t0)
     # kernel breakdown from one more step with HIP events around every launch (the
     # segment recomputes then run on the caller stream, so each interval is one launch)
     sim.timing, sim.timing_log = True, []
-    autograd.misfit_and_gradient(start, obs, sim)
+    autograd.misfit_and_gradient(v, obs, sim)
     sim.timing = False
     per = {'forward': sum(t['forward'][1] for t in sim.timing_log),
            'adjoint': sum(t['adjoint'][1] for t in sim.timing_log)}
     dominant = max(per, key=per.get)
     units = len(true) * ns
     alg = BYTES_PER_CELL_UPDATE * sim.H * sim.W * sim.geom.nt * units
+    ms_per_step = float(np.median(step_s)) * 1e3
+    log(f'autograd {ms_per_step:.2f} ms/step, kernels {per}')
     result = {
-        'metric': METRIC, 'value': units / wall, 'unit': 'shot-gradients/s', 'n_gpus': 1, 'steps': args.steps,
-        'warmup': args.warmup, 'ms_per_step': wall * 1e3, 'higher_is_better': True, 'scaling': 'weak',
-        'vs_baseline': None, 'dtype': 'f32', 'data': 'synthetic (seeded mixed-structure models)',
+        'metric': 'misfit-gradient shot-gradients/s', 'value': units / (ms_per_step * 1e-3),
+        'unit': 'shot-gradients/s', 'steps': steps, 'warmup': max(1, warmup), 'ms_per_step': ms_per_step,
+        'step_ms': [s * 1e3 for s in step_s], 'dtype': 'f32',
         'config': {'workload': 'differentiation: misfit gradient of 16 mixed 70x70 models x 5 shots, nt 1000, '
                                'autograd.Function, checkpoints every 50 steps with recompute',
                    'models': len(true), 'seed': args.seed, 'misfit': loss},
         'roofline': {'bound': 'hbm', 'kernel': dominant, 'achieved': alg / (per[dominant] * 1e-3) / 1e9,
-                     'peak': PEAK_HBM_GBS, 'unit': 'GB/s', 'frac': alg / (per[dominant] * 1e-3) / 1e9 / PEAK_HBM_GBS,
-                     'traffic': None, 'kernel_ms_per_step': per, 'algorithmic_bytes_per_direction': alg,
-                     'note': 'forward time includes the per-segment recompute (two forward passes); kernel '
-                             'times from one extra step with events around every launch, in which the '
-                             'segment recomputes run on the caller stream (the timed steps run each recompute '
-                             'on a second stream beside the previous segment\'s adjoint)'},
+                     'peak': PEAK_HBM_GBS, 'unit': 'GB/s', 'frac': None, 'traffic': None,
+                     'algorithmic_rate_frac': alg / (per[dominant] * 1e-3) / 1e9 / PEAK_HBM_GBS,
+                     'kernel_ms_per_step': per, 'algorithmic_bytes_per_direction': alg,
+                     'note': 'forward time includes the per-segment recompute (two forward passes); kernel times '
+                             'from one extra step with events around every launch (recomputes then on the '
+                             'caller stream; the timed steps run them on a side stream)'},
     }
     if not args.no_cpu:
         from oracle import fd
@@ -276,9 +422,62 @@ def run_autograd(args):
         g0 = grad[0].cpu().numpy().astype(np.float64)
         want = og.sum(1)[0].astype(np.float64)
         result['gradient_rel_l2_vs_cpu_oracle'] = float(np.linalg.norm(g0 - want) / np.linalg.norm(want))
-    print(json.dumps(result), flush=True)
+    return result
 
 
+# ----------------------------------------------------------------- refine leg
+def run_refine(args):
+    """Learned preconditioner against the plain adjoint-gradient step on a bounded
+    sample: args.refine_pairs training pairs, args.refine_models held-out models, 10
+    iterations each.  Misfit ratios, iterations for the learned step to reach the
+    plain step's final misfit, model RMS errors, and the host-CPU oracle's time for
+    one plain iteration of one model (gradient + line-search forwards)."""
+    from shotsweep_amd import refine
+    t0 = time.perf_counter()
+    res, _ = refine.run(pairs=args.refine_pairs, epochs=8, n_eval=args.refine_models, iters=10, seed=args.seed,
+                        log=lambda m: None)
+    wall = time.perf_counter() - t0
+    out = {'metric': 'learned vs plain refine misfit', 'unit': 'mean misfit ratio after k iterations',
+           'config': {'pairs': res['pairs'], 'epochs': res['epochs'], 'eval_models': res['eval_models'],
+                      'iters': res['iters'], 'seed': args.seed},
+           'plain_mean_misfit_ratio': res['plain_mean_misfit_ratio'],
+           'learned_mean_misfit_ratio': res['learned_mean_misfit_ratio'],
+           'learned_iters_to_reach_plain_final': res['learned_iters_to_reach_plain_final'],
+           'learned_fallbacks_per_iter': res['learned_fallbacks_per_iter'],
+           'model_rms_error_m_per_s': {k: res[f'model_rms_error_{k}'] for k in ('start', 'plain', 'learned')},
+           'openfwi_scores': {k: res[f'openfwi_scores_{k}'] for k in ('start', 'plain', 'learned')},
+           'time_s': {'pairs': res['time_pairs_s'], 'train': res['time_train_s'],
+                      'refine_plain': res['time_refine_plain_s'], 'refine_learned': res['time_refine_learned_s'],
+                      'total': wall},
+           'refine_iterations_per_s': res.get('refine_iterations_per_s')}
+    if not args.no_cpu:
+        from oracle import fd
+        from shotsweep_amd import synth
+        cores = len(os.sched_getaffinity(0))
+        torch.set_num_threads(cores)
+        osim = fd.OracleSim()
+        true = synth.mixed(1, seed=10_000 + args.seed)
+        start = synth.smooth(true, 8.0)
+        obs = osim.gathers(true)
+        t0 = time.perf_counter()
+        g, _ = osim.gradient(start, obs)
+        d = -g.sum(1)
+        d = d / np.abs(d).max()
+        osim.gathers(np.clip(start + 100.0 * d, 1500, 4500))     # one line-search trial
+        dt = time.perf_counter() - t0
+        from shotsweep_amd import gpu
+        gg, _ = gpu.Sim(device='cuda').gradient(start, obs)
+        want = g.sum(1).astype(np.float64)
+        out['gradient_rel_l2_vs_cpu_oracle'] = float(np.linalg.norm(gg.sum(1) - want) / np.linalg.norm(want))
+        out['cpu_baseline'] = {'value': 1.0 / dt, 'unit': 'model-iterations/s', 'cores': cores, 'kind': 'port',
+                               'sample': f'one plain iteration of one model (oracle gradient + one trial forward, '
+                                         f'FP32 eager PyTorch on {cores} threads), {dt:.1f} s'}
+        if out['refine_iterations_per_s']:
+            out['speedup_vs_cpu'] = out['refine_iterations_per_s'] / out['cpu_baseline']['value']
+    return out
+
+
+# -------------------------------------------------------------------- probes
 def copy_probe(nbytes=1 << 30, reps=3):
     """Plain device copies of known size (read nbytes, write nbytes each): the
     reference point that scales the FETCH_SIZE / WRITE_SIZE counters to bytes."""
@@ -312,6 +511,9 @@ def bandwidth_probe(nbytes=8 << 30, reps=5):
     print(json.dumps({'bandwidth_probe_bytes': nbytes, **out}), flush=True)
 
 
+LEGS = ('uneven', 'differentiation', 'refine')
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -323,10 +525,14 @@ def main():
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--cpu-models', type=int, default=20,
                     help='models in the bounded host-CPU baseline sample (x3 backgrounds x5 shots)')
-    ap.add_argument('--workload', default='batched', choices=('batched', 'mixed', 'autograd'),
-                    help='batched: 64 models of 70x70 (the headline metric); mixed: the uneven batch; '
-                         'autograd: the differentiation workload')
+    ap.add_argument('--workload', default='batched', choices=('batched', 'mixed', 'autograd', 'refine'),
+                    help='batched: the headline line with its legs; mixed / autograd / refine: one leg alone')
+    ap.add_argument('--legs', default='all', help="comma-separated subset of differentiation,uneven,refine, "
+                                                  "'all' or 'none' (batched workload only)")
+    ap.add_argument('--leg-steps', type=int, default=3, help='timed runs of each leg (median)')
     ap.add_argument('--models-per-size', type=int, default=32)
+    ap.add_argument('--refine-pairs', type=int, default=2000)
+    ap.add_argument('--refine-models', type=int, default=32)
     ap.add_argument('--copy-probe', action='store_true', help='only run 1 GiB device copies (counter scaling)')
     ap.add_argument('--bandwidth-probe', action='store_true', help='only time 8 GiB device fill / copy / sum')
     args = ap.parse_args()
@@ -339,92 +545,28 @@ def main():
               'multi-GPU runs are not part of this build', file=sys.stderr)
         sys.exit(2)
     if args.workload == 'mixed':
-        return run_mixed(args)
+        return print(json.dumps(run_mixed(args)), flush=True)
     if args.workload == 'autograd':
-        return run_autograd(args)
-    from shotsweep_amd import gpu
-
-    dev = torch.device('cuda')
-    sim = gpu.Sim(device=dev, history=args.history)
-    models, obs, obs_host, bgs, v_mute, fallbacks = workload(args.models, args.seed, sim)
-    prep = sim._prepare(bgs)
-    vm = sim._tensor(v_mute).contiguous()
-    nb = prep['nb']
-    shot_images = nb * sim.geom.ns
-    print(f'[bench] {args.models} models -> {nb} backgrounds -> {shot_images} shot-images; '
-          f'history segment {sim._segment(nb)}; surface-velocity fallbacks {fallbacks}', file=sys.stderr, flush=True)
-
-    for _ in range(args.warmup):
-        sim.images_device(prep, obs, 3, vm)
-    torch.cuda.synchronize()
-    sim.timing, sim.timing_log = True, []
-    per_kernel = {}
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        img, illum = sim.images_device(prep, obs, 3, vm)
-        for name, (n, ms) in sim.timing_log[-1].items():
-            per_kernel.setdefault(name, []).append(ms / max(n, 1))
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    sim.timing = False
-    ms_per_step = wall / args.steps * 1e3
-    value = shot_images / (wall / args.steps)
-
-    avg = {k: float(np.mean(v)) for k, v in per_kernel.items()}
-    dominant = max(avg, key=avg.get)
-    H, W, nt = sim.H, sim.W, sim.geom.nt
-    alg_bytes = BYTES_PER_CELL_UPDATE * H * W * nt * shot_images       # one cell-update per cell-step per launch
-    achieved = alg_bytes / (avg[dominant] * 1e-3) / 1e9
-    print(f'[bench] {ms_per_step:.1f} ms/step, kernels {avg}', file=sys.stderr, flush=True)
-
-    result = {
-        'metric': METRIC, 'value': value, 'unit': 'shot-images/s', 'n_gpus': 1, 'steps': args.steps,
-        'warmup': args.warmup, 'ms_per_step': ms_per_step, 'higher_is_better': True, 'scaling': 'weak',
-        'vs_baseline': None, 'dtype': 'f32', 'data': 'synthetic (seeded mixed-structure models)',
-        'config': {'workload': 'batched RTM: 64 mixed 70x70 models x 3 backgrounds x 5 shots = 960 shot-images, '
-                               'nt=1000, dt=1 ms, dx=10 m, pad 40 (150x150), 15 Hz Ricker',
-                   'models': args.models, 'shot_images': shot_images, 'seed': args.seed,
-                   'history_segment': sim._segment(nb)},
-        'roofline': {'bound': 'hbm', 'kernel': dominant, 'achieved': achieved, 'peak': PEAK_HBM_GBS, 'unit': 'GB/s',
-                     'frac': achieved / PEAK_HBM_GBS, 'traffic': measured_traffic(dominant),
-                     'traffic_source': 'profiles/hbm_traffic.json (rocprofv3 FETCH_SIZE x2.0, WRITE_SIZE x1.0)',
-                     'kernel_ms': avg, 'algorithmic_bytes_per_launch': alg_bytes,
-                     'lds': measured_lds(dominant, avg[dominant])},
-    }
-    # first-break picking for the surface-velocity anchors: HIP kernel vs the host NumPy picks
-    from shotsweep_amd import background as BG
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    BG.estimate_surface_velocity_device(obs)
-    dev_s = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    for k in range(len(obs_host)):
-        try:
-            BG.estimate_surface_velocity(obs_host[k].astype(np.float64))
-        except ValueError:
-            pass
-    result['surface_picking'] = {'models': len(obs_host), 'device_s': dev_s,
-                                 'host_s': time.perf_counter() - t0,
-                                 'note': 'device: ss_first_breaks picks + host fits; host: NumPy picks + fits'}
-    # the drop-in call with host buffers (NumPy in, NumPy out): PCIe-inclusive, never `value`
-    obs_rep = np.repeat(obs_host, 3, 0)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    sim.images(bgs, obs_rep, v_mute=v_mute)
-    host_s = time.perf_counter() - t0
-    result['host_buffers'] = {'call': 'Sim.images(v_bg, obs, v_mute) with NumPy inputs and outputs',
-                              'seconds': host_s, 'shot_images_per_s': shot_images / host_s}
-    if not args.no_cpu:
-        rows = 3 * args.cpu_models             # cpu_models models x 3 backgrounds x 5 shots
-        base, cimg, cillum, cimg64 = cpu_baseline(bgs, obs_host, v_mute, rows)
-        gimg = img[:rows].cpu().numpy()
-        err = float(np.linalg.norm(gimg - cimg) / np.linalg.norm(cimg))
-        result['cpu_baseline'] = base
-        result['speedup_vs_cpu'] = value / base['value']
-        result['image_rel_l2_vs_cpu_oracle'] = err
-        result['illum_bitwise_equal_cpu_oracle'] = bool(np.array_equal(illum[:rows].cpu().numpy(), cillum))
-        g3 = gimg[:3].astype(np.float64)
-        result['image_rel_l2_vs_fp64_oracle'] = float(np.linalg.norm(g3 - cimg64) / np.linalg.norm(cimg64))
+        return print(json.dumps(run_autograd(args)), flush=True)
+    if args.workload == 'refine':
+        return print(json.dumps(run_refine(args)), flush=True)
+    result = run_batched(args)
+    legs = LEGS if args.legs == 'all' else () if args.legs == 'none' else tuple(args.legs.split(','))
+    result['legs'] = {}
+    for leg in legs:
+        torch.cuda.empty_cache()     # return the previous workload's buffers to the device
+        t0 = time.perf_counter()
+        if leg == 'differentiation':
+            r = run_autograd(args, steps=args.leg_steps, warmup=1)
+        elif leg == 'uneven':
+            r = run_mixed(args, steps=args.leg_steps, warmup=1)
+        elif leg == 'refine':
+            r = run_refine(args)
+        else:
+            raise SystemExit(f'unknown leg {leg}')
+        r['leg_wall_s'] = time.perf_counter() - t0
+        result['legs'][leg] = r
+        log(f'leg {leg} done in {r["leg_wall_s"]:.1f} s')
     print(json.dumps(result), flush=True)
 
 

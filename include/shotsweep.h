@@ -347,7 +347,8 @@ int ss_velocity_scores_f64(int B, int H, int W, const double *pred, const double
  * Per group (arrays as in ss_forward_f32 / ss_adjoint_f32, all FP32):
  *   geom              nb rows x ns shots on (nz+2 pad) x (nx+2 pad), nt steps
  *   tile_h, tile_w, depth   own-tile shape and steps per launch; 0 = the value
- *                     ss_sweep_autotune_f32 found for this grid, else a cost model
+ *                     ss_sweep_autotune_f32 found for this grid and propagation count,
+ *                     else a cost model
  *   c2 (nb,H,W), A, B (H,W), w (nt)
  *   amp               forward: source amplitude (nb, ns); adjoint: source derivative (nb, ns)
  *   res               adjoint: muted residual (nb, ns, nt, nx)
@@ -392,8 +393,8 @@ int ss_sweep_adjoint_f32(int ngroups, const ss_sweep_group *groups, int use_grap
  * Autotuner: for each group, time up to max_trials candidate tilings (tile
  * shape and depth ranked by a halo-redundancy model; 0 = 12) as trial
  * launches of that group alone on its own buffers, write the fastest into
- * tile_h / tile_w / depth and remember it for the grid size (direction 0 =
- * forward, 1 = adjoint).  Synchronises the stream; the trial launches leave
+ * tile_h / tile_w / depth and remember it for the grid size and the group's
+ * propagation count nb * ns (direction 0 = forward, 1 = adjoint).  Synchronises the stream; the trial launches leave
  * the group's outputs and state undefined, so run the sweep afterwards.
  */
 int ss_sweep_autotune_f32(int ngroups, ss_sweep_group *groups, int direction, int max_trials, void *stream);

@@ -50,11 +50,19 @@ namespace {
 #ifndef SS_PREFETCH
 #define SS_PREFETCH 4
 #endif
+#ifndef SS_CPREFETCH
+#define SS_CPREFETCH 2
+#endif
 constexpr int kFwdThreads = 1024;            // forward: p(t-1), c2 + descriptors = 2.5 registers per cell
 constexpr int kAdjThreads = SS_ADJ_THREADS;  // adjoint: m1, m2, imaging sum = 3 registers per cell
 constexpr int kFwdMaxSlots = 24;
 constexpr int kAdjMaxSlots = 48;
 constexpr int kPrefetch = SS_PREFETCH;       // adjoint: Laplacian loads issued this many cells ahead
+constexpr int kAdjCPrefetch = SS_CPREFETCH;  // paired adjoint: c2 loads issued this many pairs ahead
+// pairs per thread of the paired adjoint on the 150 x 150 plane, rounded up to a multiple
+// of the c2 prefetch distance (pairs beyond the plane are skipped at compile time)
+constexpr int kAdjPairs150 = ((150 * 150 / 2 + kAdjThreads - 1) / kAdjThreads + kAdjCPrefetch - 1) / kAdjCPrefetch *
+                             kAdjCPrefetch;
 // slots of the 150 x 150 plane, rounded up to a multiple of the prefetch distance
 constexpr int kAdj150 = ((150 * 150 + kAdjThreads - 1) / kAdjThreads + kPrefetch - 1) / kPrefetch * kPrefetch;
 
@@ -721,13 +729,18 @@ __global__ __launch_bounds__(kAdjThreads) void adj_pair_kernel(Geom g, int t0, i
   const __amdgpu_buffer_rsrc_t rrs = make_rsrc(res + prop * g.nt * nx, (int64_t)g.nt * nx);
   const __amdgpu_buffer_rsrc_t hrs = make_rsrc(hist + prop * nslots * HW, (int64_t)nslots * HW);
   __syncthreads();
-  constexpr int D = pair_prefetch<KP>();
-  static_assert(KP % D == 0, "pair count must be a multiple of the prefetch distance");
-  float2 Lb[D], Cb[D];  // history and c2 (or c2 A) of the pairs D ahead
+  constexpr int D = pair_prefetch<KP>();       // history loads D pairs ahead
+  constexpr int DC = kAdjCPrefetch;            // c2 (or c2 A) loads DC pairs ahead
+  static_assert(KP % D == 0 && KP % DC == 0, "pair count must be a multiple of the prefetch distances");
+  float2 Lb[D], Cb[DC];  // history and c2 (or c2 A) of the pairs ahead
 #pragma unroll
   for (int d = 0; d < D; ++d) {
     const bool live = t + d * kAdjThreads < NP;
     Lb[d] = live ? buf_load2(hrs, t * 8, ((t1 - 1 - t0) * HW + 2 * d * kAdjThreads) * 4) : make_float2(0.0f, 0.0f);
+  }
+#pragma unroll
+  for (int d = 0; d < DC; ++d) {
+    const bool live = t + d * kAdjThreads < NP;
     Cb[d] = live ? buf_load2(crs, t * 8, d * kAdjThreads * 8) : make_float2(0.0f, 0.0f);
   }
   for (int it = t1 - 1; it >= t0; --it) {
@@ -798,11 +811,11 @@ __global__ __launch_bounds__(kAdjThreads) void adj_pair_kernel(Geom g, int t0, i
 #pragma unroll
     for (int j = 0; j < KP; ++j) {
       asm volatile("" : "+v"(pk[j]));
-      const float2 cv = Cb[j % D];
+      const float2 cv = Cb[j % DC];
       {
         // c2 is the same every step: after the last pair, load the first pairs of the next step
-        const int jn = j + D < KP ? j + D : j + D - KP;
-        if (t + jn * kAdjThreads < NP) Cb[j % D] = buf_load2(crs, off * 8, jn * kAdjThreads * 8);
+        const int jn = j + DC < KP ? j + DC : j + DC - KP;
+        if (t + jn * kAdjThreads < NP) Cb[j % DC] = buf_load2(crs, off * 8, jn * kAdjThreads * 8);
       }
       if (t + j * kAdjThreads < NP) {
         if (C2A) {
@@ -928,7 +941,7 @@ int adjoint_resident(const Geom &g, int t0, int t1, const float *c2, const float
     constexpr bool F = decltype(fixed)::value;
     // the fixed 150 x 150 plane runs the paired kernel (22 pairs per thread), with the
     // c2 * A plane when the caller has one
-    constexpr int KP = (150 * 150 / 2 + kAdjThreads - 1) / kAdjThreads;
+    constexpr int KP = kAdjPairs150;
     const bool pre = F && c2a != nullptr;
     auto kern = F ? (pre ? adj_pair_kernel<KP, 70, 70, 40, 1, true> : adj_pair_kernel<KP, 70, 70, 40, 1, false>)
                   : adj_resident_kernel<decltype(kc)::value, F ? 70 : 0, F ? 70 : 0, F ? 40 : 0, F ? 1 : -1>;

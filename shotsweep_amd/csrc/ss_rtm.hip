@@ -398,7 +398,7 @@ int rtm(const ss_geom *gg, const ss_rtm_args *a, void *ws, int64_t ws_bytes, hip
         SS_TRY(copy_rows<float>(L.tamp, 0, amp, 0, 1, nprop, st));
         float *fst = L.state;  // 2 buffers x (p0, p1)
         // the autotuner's trial launches overwrite the state: tune before the first segment
-        if (!noalloc && !sweep_tuned(0, g.H, g.W, g.pad)) {
+        if (!noalloc && !sweep_tuned(0, g.H, g.W, g.pad, nprop)) {
           ss_sweep_group q = tiled_group(*gg, L, A, B, w, 0, 0, segment, 0, 0, fst, L.seg[0]);
           SS_TRY(sweep_autotune(1, &q, 0, 0, st));
         }
@@ -499,7 +499,7 @@ int rtm(const ss_geom *gg, const ss_rtm_args *a, void *ws, int64_t ws_bytes, hip
       SS_TRY(copy_rows<float>(L.tc2, 0, c2, 0, 1, g.nb * HW, st));
       SS_TRY(copy_rows<float>(L.tsd, 0, sd, 0, 1, nprop, st));
       float *fst = L.plane(0, nprop, HW), *ast = L.plane(4, nprop, HW);
-      if (!noalloc && !sweep_tuned(0, g.H, g.W, g.pad)) {
+      if (!noalloc && !sweep_tuned(0, g.H, g.W, g.pad, nprop)) {
         int t0, t1;
         span(nck - 1, &t0, &t1);
         ss_sweep_group q = tiled_group(*gg, L, A, B, w, 0, t0, t1, 0, 0, fst, L.seg[0]);
@@ -524,7 +524,7 @@ int rtm(const ss_geom *gg, const ss_rtm_args *a, void *ws, int64_t ws_bytes, hip
         return SS_OK;
       };
       SS_TRY(recompute(nck - 1));
-      if (!noalloc && !sweep_tuned(1, g.H, g.W, g.pad)) {
+      if (!noalloc && !sweep_tuned(1, g.H, g.W, g.pad, nprop)) {
         // the adjoint trials read the last segment's history; the first real adjoint
         // call (resume 0) then overwrites the outputs the trials left
         if (overlap) SS_TRY(hip_status(hipStreamWaitEvent(st, R->ready[(nck - 1) & 1], 0), "ready wait"));

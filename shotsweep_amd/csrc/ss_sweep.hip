@@ -755,11 +755,14 @@ std::vector<TileChoice> candidates(int H, int W, int pad, int dir) {
 }
 
 std::mutex g_mutex;
-std::map<std::string, TileChoice> g_tuned;  // "dir:H:W:pad" -> tuned tiling
+std::map<std::string, TileChoice> g_tuned;  // "dir:H:W:pad:propagations" -> tuned tiling
 
-std::string tune_key(int dir, int H, int W, int pad) {
-  char buf[64];
-  snprintf(buf, sizeof buf, "%d:%d:%d:%d", dir, H, W, pad);
+// Tuned tilings are kept per direction, grid and propagation count: a tiling
+// that fills the GPU with a few hundred propagations is not the one that
+// spreads 80 of them over all CUs.
+std::string tune_key(int dir, int H, int W, int pad, int64_t nprop) {
+  char buf[96];
+  snprintf(buf, sizeof buf, "%d:%d:%d:%d:%lld", dir, H, W, pad, (long long)nprop);
   return buf;
 }
 
@@ -791,7 +794,7 @@ int fill_group(const ss_sweep_group &in, int dir, SGroup *o, char *err, size_t e
   TileChoice tc{in.tile_h, in.tile_w, in.depth};
   if (tc.th <= 0 || tc.tw <= 0 || tc.T <= 0) {
     std::lock_guard<std::mutex> lock(g_mutex);
-    auto it = g_tuned.find(tune_key(dir, g.H, g.W, g.pad));
+    auto it = g_tuned.find(tune_key(dir, g.H, g.W, g.pad, (int64_t)g.nb * g.ns));
     if (it != g_tuned.end()) {
       tc = it->second;
     } else {
@@ -1157,9 +1160,9 @@ int sweep_run(int ngroups, const ss_sweep_group *groups, int dir, int use_graph,
   return run_plan(p, use_graph, st);
 }
 
-bool sweep_tuned(int dir, int H, int W, int pad) {
+bool sweep_tuned(int dir, int H, int W, int pad, int64_t nprop) {
   std::lock_guard<std::mutex> lock(g_mutex);
-  return g_tuned.count(tune_key(dir, H, W, pad)) != 0;
+  return g_tuned.count(tune_key(dir, H, W, pad, nprop)) != 0;
 }
 
 int sweep_launches(const ss_sweep_group &group, int dir) {
@@ -1241,7 +1244,7 @@ int sweep_autotune(int ngroups, ss_sweep_group *groups, int dir, int max_trials,
       }
     }
     std::lock_guard<std::mutex> lock(g_mutex);
-    g_tuned[tune_key(dir, g.H, g.W, g.pad)] = pick;
+    g_tuned[tune_key(dir, g.H, g.W, g.pad, (int64_t)g.nb * g.ns)] = pick;
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);

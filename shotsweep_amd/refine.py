@@ -201,9 +201,16 @@ def run(pairs=20000, epochs=8, n_eval=256, iters=10, seed=0, log=print):
     start = synth.smooth(true, 8.0)
     prob = Problem(sim, true)
     log('[refine] plain')
+    torch.cuda.synchronize()
+    t2 = time.time()
     hp, vp, _ = refine(prob, start, iters, None, log)
+    torch.cuda.synchronize()
+    t_plain = time.time() - t2
     log('[refine] learned')
+    t3 = time.time()
     hl, vl, falls = refine(prob, start, iters, net, log)
+    torch.cuda.synchronize()
+    t_learned = time.time() - t3
     plain_ratio = (hp / hp[:, :1]).mean(0)
     learned_ratio = (hl / hl[:, :1]).mean(0)
     tt = torch.tensor(true, dtype=torch.float32, device=sim.device)
@@ -216,6 +223,8 @@ def run(pairs=20000, epochs=8, n_eval=256, iters=10, seed=0, log=print):
     result = {
         'pairs': pairs, 'epochs': epochs, 'eval_models': n_eval, 'iters': iters, 'seed': seed,
         'train_loss': losses, 'time_pairs_s': t_pairs, 'time_train_s': t_train,
+        'time_refine_plain_s': t_plain, 'time_refine_learned_s': t_learned,
+        'refine_iterations_per_s': n_eval * iters / t_plain,
         'plain_mean_misfit_ratio': plain_ratio.tolist(),
         'learned_mean_misfit_ratio': learned_ratio.tolist(),
         'learned_fallbacks_per_iter': falls,
