@@ -281,7 +281,9 @@ class OracleSim:
                 illum += L[..., p:p + g.nz, p:p + g.nx] ** 2
                 m2, m1 = m1, mu
         vp = self._tensor(np.pad(np.asarray(v_bg), ((0, 0), (p, p), (p, p)), mode='edge'))[:, None]
-        grad = fold_edge_padding(grad_acc * 2 * vp * (g.dt / g.dx) ** 2, p, g.nz, g.nx)
+        scaled = grad_acc * 2 * vp * (g.dt / g.dx) ** 2
+        self.last_scaled = scaled      # the unfolded terms, kept for rounding bounds in tests
+        grad = fold_edge_padding(scaled, p, g.nz, g.nx)
         grad[:, self.si, g.row, torch.tensor(list(g.src))] += source_grad
         return grad, illum
 

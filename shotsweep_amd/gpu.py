@@ -45,7 +45,7 @@ def resolve_device(name='auto'):
     device = torch.device(name)
     if device.type == 'cuda' and not torch.cuda.is_available():
         raise RuntimeError('CUDA was requested, but this PyTorch installation has no available CUDA device')
-    if device.type == 'mps':
+    if device.type == 'mps' and not torch.backends.mps.is_available():
         raise RuntimeError('MPS was requested, but no MPS device is available')
     if device.type not in ('cpu', 'cuda', 'mps'):
         raise ValueError(f'unsupported device: {name}')

@@ -57,3 +57,61 @@ def test_depth_trend_and_backgrounds_closed_form():
     np.testing.assert_allclose(bgs[1, :, 7], np.clip(2000 + trend, 1500, 4500))
     np.testing.assert_allclose(bgs[2, -1, 0], min(2000 + 1.4 * 1500, 4500))
     assert np.all(bgs == bgs[:, :, :1])          # laterally constant
+
+
+# --------------------------------------------- values pinned to the reference
+NAMES = ['homogeneous_1500', 'homogeneous_2500', 'homogeneous_4500', 'refractor_depth5', 'refractor_depth2']
+
+
+def _surface_golden():
+    import json
+    from pathlib import Path
+    path = Path(__file__).resolve().parent / 'golden' / 'reference_surface_velocity.json'
+    return json.loads(path.read_text())
+
+
+def _digest(a):
+    import hashlib
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def _same_estimate(got, want):
+    """The dict of the reference's estimate_surface_velocity, every value equal
+    (exact float equality: the same picks, medians and fits), or its error."""
+    if 'error' in want:
+        assert isinstance(got, ValueError) and str(got) == want['error'], (got, want)
+        return
+    assert not isinstance(got, Exception), got
+    assert set(got) == set(want)
+    for k, v in want.items():
+        assert got[k] == v or (np.isnan(v) and np.isnan(got[k])), (k, got[k], v)
+
+
+def _host_estimate(obs):
+    try:
+        return BG.estimate_surface_velocity(obs)
+    except ValueError as error:
+        return error
+
+
+def test_estimates_equal_reference_on_golden_gathers(golden):
+    """The reference's estimates of the golden flat-layer gathers (FP32 and FP64),
+    recorded by tests/golden/make_surface_golden.py, reproduced exactly by the host
+    estimator on the same gathers."""
+    ref = _surface_golden()['golden']
+    for tag in ('f32', 'f64'):
+        obs = golden[f'obs_{tag}'][0]
+        assert _digest(obs) == ref[tag]['gathers_sha256']
+        _same_estimate(_host_estimate(obs.astype(np.float64)), ref[tag]['estimate'])
+
+
+def test_estimates_equal_reference_on_its_test_models(observed):
+    """The five models of the reference's surface-velocity tests: the oracle's FP32
+    gathers are the reference's bytes (SHA-256), and the host estimator returns the
+    reference's dicts exactly on them as FP32 and as float64 (and its error message
+    for the unresolved refractor)."""
+    ref = _surface_golden()['reference_tests']
+    for name, obs in zip(NAMES, observed):
+        assert _digest(obs) == ref[name]['gathers_sha256'], name
+        _same_estimate(_host_estimate(obs), ref[name]['estimate'])
+        _same_estimate(_host_estimate(obs.astype(np.float64)), ref[name]['estimate_f64'])

@@ -93,6 +93,60 @@ def test_gradient_matches_reference(golden, impl, dtype, tag):
     assert e < (F32_TOL if tag == 'f32' else F64_TOL)
 
 
+def _fold_terms(pad, nz, nx):
+    """Terms each model pixel of fold_edge_padding sums: 1 own value, pad per padding
+    strip, pad^2 per corner block (gpu.py:56-67)."""
+    n = np.ones((nz, nx))
+    n[0, :] += pad
+    n[-1, :] += pad
+    n[:, 0] += pad
+    n[:, -1] += pad
+    for i in (0, -1):
+        for j in (0, -1):
+            n[i, j] += pad * pad
+    return n
+
+
+def test_gradient_interior_bit_identical_edges_within_fold_bound(golden):
+    """FP32 gradient against the reference's: every pixel off the model's edge rows and
+    columns (including the source pixels of the three inner shots, with their
+    source-amplitude term) is the reference's to the bit; only the fold's padding-strip
+    and corner-block sums run in another order, so each edge pixel is within the
+    summation bound 2 n eps sum|terms| of that pixel (n terms; sum|terms| folded from
+    the oracle's unfolded imaging sum)."""
+    case('gradient interior bitwise, edge pixels within the fold rounding bound')
+    bg, obs = golden['backgrounds'][:1], golden['obs_f32']
+    grad, illum = gpu.Sim(device='cuda').gradient(bg, obs)
+    want = golden['grad_f32']
+    inner = np.zeros(want.shape, bool)
+    inner[..., 1:-1, 1:-1] = True
+    np.testing.assert_array_equal(grad[inner], want[inner])
+    osim = fd.OracleSim()
+    og, _ = osim.gradient(bg, obs)
+    np.testing.assert_array_equal(og, want)          # the oracle is the reference here
+    mag = fd.fold_edge_padding(osim.last_scaled.abs(), 40, 70, 70).numpy().astype(np.float64)
+    bound = 2 * _fold_terms(40, 70, 70) * 2.0 ** -24 * mag + 2.0 ** -23 * np.abs(want)
+    err = np.abs(grad.astype(np.float64) - want)
+    worst = float((err / bound)[~inner].max())
+    print(f'  edge pixels: worst error / bound {worst:.3f}; interior bit-identical')
+    assert np.all(err[~inner] <= bound[~inner])
+
+
+def test_images_interior_bit_identical(golden):
+    """FP32 images (mute, 3 backgrounds): the Laplacian filter of a pixel reads
+    pixels up to 2 away, so every pixel at least 3 from the model's edge is the
+    reference's to the bit; the rest within 1e-5 relative L2."""
+    case('images interior bitwise')
+    obs = golden['obs_f32']
+    img, illum = gpu.Sim(device='cuda').images(golden['backgrounds'], np.repeat(obs, 3, 0), v_mute=golden['v_mute'])
+    want = golden['img_f32']
+    inner = np.zeros(want.shape, bool)
+    inner[..., 3:-3, 3:-3] = True
+    np.testing.assert_array_equal(img[inner], want[inner])
+    np.testing.assert_array_equal(illum, golden['illum_f32'])
+    assert rel(img[~inner], want[~inner]) < F32_TOL
+
+
 def test_checkpoint_recompute_equals_full_history(golden):
     case('checkpoint vs full history bitwise')
     obs = np.repeat(golden['obs_f32'], 3, 0)

@@ -68,3 +68,24 @@ def test_scores_match_oracle(shape):
     d = scores.metrics(pred[0], true[0])
     w = oscores.metrics(pred[0], true[0])
     assert set(d) == {'mae', 'rmse', 'ssim'} and all(abs(d[k] - w[k]) <= 1e-12 * abs(w[k]) for k in d)
+
+
+def test_device_estimates_equal_reference_values():
+    """GPU gathers of the reference's five surface-velocity test models are the
+    reference's bytes, and the GPU picks plus host fits on them as float64 (as
+    the harness reads gathers) return the reference's recorded float64 estimates
+    exactly (tests/golden/make_surface_golden.py)."""
+    from test_background import NAMES, _digest, _same_estimate, _surface_golden
+    print('\n[case] surface-velocity estimates vs reference values', flush=True)
+    ref = _surface_golden()['reference_tests']
+    models = [np.full((70, 70), v, dtype=float) for v in (1500, 2500, 4500)]
+    for depth in (5, 2):
+        m = np.full((70, 70), 1500.0)
+        m[depth:] = 4500.0
+        models.append(m)
+    obs = gpu.Sim(device='cuda').forward(np.array(models))[0].contiguous()
+    host = obs.cpu().numpy()
+    for name, o in zip(NAMES, host):
+        assert _digest(o) == ref[name]['gathers_sha256'], name
+    for name, est in zip(NAMES, BG.estimate_surface_velocity_device(obs.double())):
+        _same_estimate(est, ref[name]['estimate_f64'])

@@ -184,3 +184,44 @@ def test_uneven_workload_sizes_bit_identical_to_stepwise():
         print(f'  group {k}: tiles {ms.tiles[k]}', flush=True)
         assert torch.equal(out[k][0], img), f'images group {k}'
         assert torch.equal(out[k][1], illum), f'illumination group {k}'
+
+
+def _sizes_fixture():
+    import importlib.util
+    import sys
+    from pathlib import Path
+    here = Path(__file__).resolve().parent / 'golden'
+    spec = importlib.util.spec_from_file_location('make_oracle_sizes', here / 'make_oracle_sizes.py')
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules.setdefault('make_oracle_sizes', mod)
+    spec.loader.exec_module(mod)
+    with np.load(here / 'oracle_uneven_sizes.npz') as d:
+        return mod, {k: d[k] for k in d.files}
+
+
+@pytest.mark.parametrize('size', [0, 1])
+def test_uneven_sizes_match_oracle_at_full_size(size):
+    """One background row at each larger grid of the uneven batch at its real size
+    (100x140, nt 1500, 7 shots; 200x200, nt 2000, 10 shots) against the oracle's
+    fixture (tests/golden/make_oracle_sizes.py): gathers and illumination the same
+    bytes (SHA-256), images within 1e-5 relative L2, through both the drop-in
+    Sim and the mixed sweep that runs the uneven batch."""
+    mod, fix = _sizes_fixture()
+    nz, nx, nt, ns, seed = mod.SIZES[size]
+    tag = f'{nz}x{nx}'
+    case(f'uneven size {tag} nt {nt} ns {ns} vs oracle fixture')
+    model, bg, vm = mod.case_inputs(nz, nx, seed)
+    acq = Acquisition(nz=nz, nx=nx, nt=nt, ns=ns)
+    sim = gpu.Sim(geom=acq)
+    obs = sim.forward(model)[0].contiguous()
+    assert mod.digest(obs.cpu().numpy()) == str(fix[f'{tag}_gathers_sha256'])
+    img, illum = sim.images(bg, obs.cpu().numpy(), v_mute=vm)
+    assert mod.digest(illum) == str(fix[f'{tag}_illum_sha256'])
+    e = rel(img, fix[f'{tag}_img'])
+    ms = MixedSweep([acq])
+    (mimg, mill), = ms.images_device([{'v_bg': bg, 'obs': obs, 'obs_repeat': 1,
+                                       'v_mute': torch.tensor(vm, dtype=torch.float32, device=DEV)}])
+    assert mod.digest(mill.cpu().numpy()) == str(fix[f'{tag}_illum_sha256'])
+    em = rel(mimg.cpu().numpy(), fix[f'{tag}_img'])
+    print(f'  image rel L2 vs oracle: Sim {e:.3e}, mixed sweep {em:.3e}')
+    assert e < 1e-5 and em < 1e-5
