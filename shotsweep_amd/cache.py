@@ -1,22 +1,24 @@
-"""On-disk formats of the imaging caches, compatible with the reference's readers.
+"""The cache files on either side of the imaging path, in the format the reference's
+readers accept.
 
-The reference harness (openfwi/2_process.py:26-109, 137-218) writes two caches
-that its training and test scripts consume (3_train.py:166-189):
+Format (shared with the reference because its training and test scripts read these
+files; the code here is this project's own):
 
-* the calibration cache: ``source_wavelet.npy``, ``depth_trend.npy`` and
-  ``calibration_cache.json`` {fingerprint, provenance, wavelet_sha256,
-  trend_sha256};
-* one image cache per family, ``inputs/<family>.npz`` with ``images`` float16
-  (n, 3, 5, 70, 70), ``surface_velocity``, ``surface_confidence``,
-  ``surface_fit_rmse_ms`` (float32, n), ``surface_possible_refraction`` (bool,
-  n) and ``cache_provenance`` (a JSON string with its own fingerprint).
+* calibration, in a work directory (written by 2_process.py:147-161):
+  ``source_wavelet.npy`` and ``depth_trend.npy`` (float64), and
+  ``calibration_cache.json`` = {fingerprint, provenance, wavelet_sha256, trend_sha256};
+* one image cache per family, ``inputs/<family>.npz`` (2_process.py:187-218, read by
+  3_train.py:166-189): the arrays of ``FAMILY_SCHEMA`` plus ``cache_provenance``, a
+  JSON string (sorted keys) whose ``fingerprint`` is the fingerprint of its other keys;
+* a fingerprint is the SHA-256 hex digest of canonical JSON (sorted keys, ',' and ':'
+  separators); a file digest is the SHA-256 of its bytes.
 
-The fingerprint is the SHA-256 of the provenance serialised as canonical JSON
-(sorted keys, no spaces); file digests are SHA-256 of the bytes.  Files are
-written atomically (temporary file in the same directory, fsync, rename), so
-a reader never sees a partial cache.  ``family_arrays`` assembles the cache
-arrays from ``sweep.sweep_batch`` results.
+Every file is replaced atomically (``replaced_atomically``), so a reader sees the
+old file or the new one, never a partial write.  ``FamilyCache`` collects the
+results of ``sweep.sweep_batch`` batch by batch and writes the family's file once
+all models are in.
 """
+import contextlib
 import hashlib
 import json
 import os
@@ -29,145 +31,170 @@ from .background import RHOS
 
 CACHE_SCHEMA = 2
 
+# name, dtype and per-model shape of every array of a family cache (2_process.py:187-191, 216-218)
+FAMILY_SCHEMA = (
+    ('images', np.float16, (len(RHOS), 5, 70, 70)),
+    ('surface_velocity', np.float32, ()),
+    ('surface_confidence', np.float32, ()),
+    ('surface_fit_rmse_ms', np.float32, ()),
+    ('surface_possible_refraction', np.bool_, ()),
+)
+# arrays whose shape and finiteness the reference's validator checks (2_process.py:97-109)
+CHECKED = ('images', 'surface_velocity', 'surface_confidence')
 
-def file_digest(path):
-    """SHA-256 hex digest of a file, read in 1 MiB blocks."""
+
+def sha256_of_file(path, block=1 << 20):
+    """SHA-256 hex digest of a file's bytes."""
     h = hashlib.sha256()
     with open(path, 'rb') as f:
-        while True:
-            block = f.read(1 << 20)
-            if not block:
-                break
-            h.update(block)
+        for chunk in iter(lambda: f.read(block), b''):
+            h.update(chunk)
     return h.hexdigest()
 
 
 def fingerprint(value):
-    """SHA-256 of canonical JSON (sorted keys, ',' and ':' separators)."""
-    text = json.dumps(value, sort_keys=True, separators=(',', ':'))
-    return hashlib.sha256(text.encode()).hexdigest()
+    """SHA-256 hex digest of ``value`` as canonical JSON."""
+    return hashlib.sha256(json.dumps(value, sort_keys=True, separators=(',', ':')).encode()).hexdigest()
 
 
-def _atomic(path, write):
+def with_fingerprint(record):
+    """A copy of ``record`` with its own fingerprint (of every other key) added."""
+    body = {k: v for k, v in record.items() if k != 'fingerprint'}
+    return dict(body, fingerprint=fingerprint(body))
+
+
+@contextlib.contextmanager
+def replaced_atomically(path):
+    """Yield a binary file whose contents replace ``path`` in one rename when the block
+    exits cleanly (flushed and fsync'd first); if the block raises, ``path`` keeps its
+    previous contents and the temporary file is removed."""
     path = Path(path)
     path.parent.mkdir(parents=True, exist_ok=True)
-    fd, tmp = tempfile.mkstemp(prefix=f'.{path.name}.', suffix='.tmp', dir=path.parent)
+    handle, temporary = tempfile.mkstemp(dir=path.parent, prefix=f'.{path.name}.', suffix='.tmp')
+    done = False
     try:
-        with os.fdopen(fd, 'wb') as f:
-            write(f)
+        with os.fdopen(handle, 'wb') as f:
+            yield f
             f.flush()
             os.fsync(f.fileno())
-        os.replace(tmp, path)
+        os.replace(temporary, path)
+        done = True
     finally:
-        if os.path.exists(tmp):
-            os.unlink(tmp)
-
-
-def save_npy(path, values):
-    _atomic(path, lambda f: np.save(f, values, allow_pickle=False))
-
-
-def save_npz(path, **values):
-    _atomic(path, lambda f: np.savez(f, **values))
-
-
-def save_json(path, value):
-    _atomic(path, lambda f: f.write(json.dumps(value, sort_keys=True, indent=2).encode()))
+        if not done and os.path.exists(temporary):
+            os.remove(temporary)
 
 
 # ------------------------------------------------------------ calibration
-def write_calibration(work, wavelet, trend, provenance):
-    """Write the calibration cache; returns its metadata dict."""
-    work = Path(work)
-    wavelet, trend = np.asarray(wavelet), np.asarray(trend)
-    if not np.isfinite(wavelet).all() or not np.any(wavelet) or not np.isfinite(trend).all():
-        raise ValueError('calibration produced invalid source wavelet or depth trend')
-    save_npy(work / 'source_wavelet.npy', wavelet)
-    save_npy(work / 'depth_trend.npy', trend)
-    meta = {'fingerprint': fingerprint(provenance), 'provenance': provenance,
-            'wavelet_sha256': file_digest(work / 'source_wavelet.npy'),
-            'trend_sha256': file_digest(work / 'depth_trend.npy')}
-    save_json(work / 'calibration_cache.json', meta)
-    return meta
+class CalibrationCache:
+    """The calibrated source wavelet and depth trend of a work directory."""
 
+    WAVELET, TREND, META = 'source_wavelet.npy', 'depth_trend.npy', 'calibration_cache.json'
 
-def calibration_valid(work, expected_fingerprint):
-    """True when the calibration cache matches the fingerprint and both files' digests."""
-    work = Path(work)
-    try:
-        with open(work / 'calibration_cache.json') as f:
-            meta = json.load(f)
-        return (meta['fingerprint'] == expected_fingerprint
-                and meta['wavelet_sha256'] == file_digest(work / 'source_wavelet.npy')
-                and meta['trend_sha256'] == file_digest(work / 'depth_trend.npy'))
-    except (OSError, ValueError, KeyError):
-        return False
+    def __init__(self, work):
+        self.work = Path(work)
+
+    def metadata(self):
+        """The parsed metadata file, or None when it is missing or unreadable."""
+        try:
+            return json.loads((self.work / self.META).read_text())
+        except (OSError, ValueError):
+            return None
+
+    def matches(self, expected_fingerprint):
+        """Whether the cache was written for this provenance fingerprint and both arrays
+        still have the digests recorded with it."""
+        meta = self.metadata()
+        if not isinstance(meta, dict):
+            return False
+        try:
+            return (meta['fingerprint'] == expected_fingerprint
+                    and meta['wavelet_sha256'] == sha256_of_file(self.work / self.WAVELET)
+                    and meta['trend_sha256'] == sha256_of_file(self.work / self.TREND))
+        except (OSError, KeyError):
+            return False
+
+    def store(self, wavelet, trend, provenance):
+        """Write both arrays, then the metadata that names their digests; returns it."""
+        wavelet, trend = np.asarray(wavelet), np.asarray(trend)
+        if not (np.isfinite(wavelet).all() and np.any(wavelet) and np.isfinite(trend).all()):
+            raise ValueError('calibration produced invalid source wavelet or depth trend')
+        for name, values in ((self.WAVELET, wavelet), (self.TREND, trend)):
+            with replaced_atomically(self.work / name) as f:
+                np.save(f, values, allow_pickle=False)
+        meta = {'fingerprint': fingerprint(provenance), 'provenance': provenance,
+                'wavelet_sha256': sha256_of_file(self.work / self.WAVELET),
+                'trend_sha256': sha256_of_file(self.work / self.TREND)}
+        with replaced_atomically(self.work / self.META) as f:
+            f.write(json.dumps(meta, sort_keys=True, indent=2).encode())
+        return meta
+
+    def load(self):
+        """(wavelet, trend, metadata) as stored."""
+        return (np.load(self.work / self.WAVELET, allow_pickle=False),
+                np.load(self.work / self.TREND, allow_pickle=False), self.metadata())
+
+    def ensure(self, provenance, compute):
+        """The cached calibration when it matches ``provenance``; otherwise
+        ``compute()`` -> (wavelet, trend) is stored first."""
+        if not self.matches(fingerprint(provenance)):
+            self.store(*compute(), provenance)
+        return self.load()
 
 
 def calibrated(work, provenance, models, gathers, training_models):
-    """Source wavelet and depth trend, from a valid calibration cache or computed and
-    cached (2_process.py:137-161): the wavelet from ``gpu.calibrate`` on the
-    calibration models and their gathers, the trend from the training models.
-    Returns (wavelet, trend, metadata)."""
+    """Source wavelet and depth trend of a work directory (2_process.py:147-161): reused
+    when the calibration cache matches ``provenance``, else the wavelet comes from
+    ``gpu.calibrate`` on the calibration models and their gathers and the trend from
+    the training models, and both are cached.  Returns (wavelet, trend, metadata)."""
     from . import gpu
     from .background import depth_trend
-    work = Path(work)
-    if not calibration_valid(work, fingerprint(provenance)):
-        wavelet = gpu.calibrate(models, gathers)
-        trend = depth_trend(training_models)
-        write_calibration(work, wavelet, trend, provenance)
-    with open(work / 'calibration_cache.json') as f:
-        meta = json.load(f)
-    return (np.load(work / 'source_wavelet.npy', allow_pickle=False),
-            np.load(work / 'depth_trend.npy', allow_pickle=False), meta)
+    return CalibrationCache(work).ensure(provenance,
+                                         lambda: (gpu.calibrate(models, gathers), depth_trend(training_models)))
 
 
 # ------------------------------------------------------------ image cache
-def family_arrays(n, results, nb=len(RHOS), ns=5, nz=70, nx=70):
-    """Cache arrays for n models from (indices, sweep_batch result) pairs."""
-    images = np.zeros((n, nb, ns, nz, nx), np.float16)
-    surface = np.zeros(n, np.float32)
-    confidence = np.zeros(n, np.float32)
-    fit_rmse = np.zeros(n, np.float32)
-    refraction = np.zeros(n, bool)
-    for ks, r in results:
-        ks = list(ks)
-        images[ks] = r['images']
-        surface[ks] = r['surface_velocity']
-        confidence[ks] = r['surface_confidence']
-        fit_rmse[ks] = r['surface_fit_rmse_ms']
-        refraction[ks] = r['surface_possible_refraction']
-    return {'images': images, 'surface_velocity': surface, 'surface_confidence': confidence,
-            'surface_fit_rmse_ms': fit_rmse, 'surface_possible_refraction': refraction}
+def family_provenance(calibration, family, data_identity, n_total, n_train, n_val):
+    """Provenance record of a family cache (its keys are the format's,
+    2_process.py:176-183), with its fingerprint; ``calibration`` is the calibration
+    cache's metadata."""
+    return with_fingerprint({
+        'schema': CACHE_SCHEMA, 'family': family, 'data': data_identity,
+        'calibration_fingerprint': calibration['fingerprint'],
+        'calibration_wavelet_sha256': calibration['wavelet_sha256'],
+        'calibration_trend_sha256': calibration['trend_sha256'],
+        'n_total': n_total, 'n_train': n_train, 'n_val': n_val})
 
 
-def input_provenance(calibration_meta, family, data_identity, n_total, n_train, n_val):
-    """Provenance of a family cache, with its fingerprint (2_process.py:173-181)."""
-    prov = {'schema': CACHE_SCHEMA, 'calibration_fingerprint': calibration_meta['fingerprint'],
-            'calibration_wavelet_sha256': calibration_meta['wavelet_sha256'],
-            'calibration_trend_sha256': calibration_meta['trend_sha256'],
-            'family': family, 'data': data_identity, 'n_total': n_total, 'n_train': n_train, 'n_val': n_val}
-    prov['fingerprint'] = fingerprint(prov)
-    return prov
+class FamilyCache:
+    """The image cache of one family of ``n_total`` models, filled batch by batch."""
 
+    def __init__(self, path, n_total):
+        self.path = Path(path)
+        self.n_total = int(n_total)
+        self.arrays = {name: np.zeros((self.n_total,) + shape, dtype) for name, dtype, shape in FAMILY_SCHEMA}
 
-def write_family(path, arrays, provenance):
-    """Write one family's image cache atomically."""
-    save_npz(path, **arrays, cache_provenance=np.array(json.dumps(provenance, sort_keys=True)))
+    def add(self, indices, result):
+        """Store a ``sweep.sweep_batch`` result for the models ``indices``."""
+        indices = list(indices)
+        for name, dtype, _ in FAMILY_SCHEMA:
+            self.arrays[name][indices] = np.asarray(result[name], dtype)
 
+    def write(self, provenance):
+        """Replace the family's file atomically with the arrays and the provenance."""
+        text = json.dumps(provenance, sort_keys=True)
+        with replaced_atomically(self.path) as f:
+            np.savez(f, **self.arrays, cache_provenance=np.array(text))
 
-def inputs_valid(path, expected_fingerprint, n_total, nb=len(RHOS)):
-    """True when a family cache has the fingerprint, shapes and finite values the
-    reference's reader requires (2_process.py:97-109)."""
-    try:
-        with np.load(path, allow_pickle=False) as c:
-            meta = json.loads(str(c['cache_provenance'].item()))
-            return (meta['fingerprint'] == expected_fingerprint
-                    and c['images'].shape == (n_total, nb, 5, 70, 70)
-                    and c['surface_velocity'].shape == (n_total,)
-                    and c['surface_confidence'].shape == (n_total,)
-                    and np.isfinite(c['images']).all()
-                    and np.isfinite(c['surface_velocity']).all()
-                    and np.isfinite(c['surface_confidence']).all())
-    except (OSError, ValueError, KeyError, EOFError):
-        return False
+    @staticmethod
+    def matches(path, expected_fingerprint, n_total):
+        """Whether the file at ``path`` is a readable family cache for this fingerprint
+        with ``n_total`` models and finite values, the conditions under which the
+        reference reuses a cache (2_process.py:97-109)."""
+        shapes = {name: (n_total,) + shape for name, _, shape in FAMILY_SCHEMA}
+        try:
+            with np.load(path, allow_pickle=False) as c:
+                if json.loads(str(c['cache_provenance'].item())).get('fingerprint') != expected_fingerprint:
+                    return False
+                return all(c[name].shape == shapes[name] and bool(np.isfinite(c[name]).all()) for name in CHECKED)
+        except (OSError, ValueError, KeyError, EOFError):
+            return False

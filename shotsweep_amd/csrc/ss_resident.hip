@@ -275,7 +275,7 @@ __global__ __launch_bounds__(kFwdThreads) void fwd_resident_kernel(Geom g, int t
     const int slot = it - t0;
     opaque(pk);
     if (rec_thread && it > t0)  // p(it) after injection is the sample of step it-1
-      buf_store(P[rec0 + t], grs, t * 4, (it - 1) * nx * 4);
+      for (int x = t; x < nx; x += kFwdThreads) buf_store(P[rec0 + x], grs, x * 4, (it - 1) * nx * 4);
     if (ckpt && slot % ckpt_every == 0) {
       const int base = (slot / ckpt_every) * 2 * HW;
 #pragma unroll
@@ -318,7 +318,8 @@ __global__ __launch_bounds__(kFwdThreads) void fwd_resident_kernel(Geom g, int t
     if (src_owner) P[srci] = P[srci] + w[it] * ampv;  // gpu.py:114, added after damping
     __syncthreads();
   }
-  if (rec_thread) buf_store(P[rec0 + t], grs, t * 4, (t1 - 1) * nx * 4);
+  if (rec_thread)
+    for (int x = t; x < nx; x += kFwdThreads) buf_store(P[rec0 + x], grs, x * 4, (t1 - 1) * nx * 4);
   float *p0o = p0g + prop * HW;
   float *p1o = p1g + prop * HW;
 #pragma unroll

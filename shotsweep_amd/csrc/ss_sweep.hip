@@ -741,7 +741,9 @@ std::vector<TileChoice> candidates(int H, int W, int pad, int dir) {
         if (W % 2 == 0 && tw < W) tw += tw & 1;  // even widths keep cell pairs inside tiles
         if (ntx > 1 && tw < 16) break;
         const TileChoice tc{th, tw, T};
-        if (ext_cells(H, W, tc) > cap ||
+        // one thread per receiver column of a tile: at most kThreads columns
+        if (tw > kThreads ||
+            ext_cells(H, W, tc) > cap ||
             lds_floats(ext_cells(H, W, tc), ext_width(W, tc), pad, dir, nullptr, nullptr, nullptr) > kMaxLdsFloats)
           continue;
         c.push_back({redundancy(H, W, tc) * (1.0 + 2.0 / T), tc});
@@ -810,6 +812,10 @@ int fill_group(const ss_sweep_group &in, int dir, SGroup *o, char *err, size_t e
   tc.tw = std::min(tc.tw, g.W);
   if (g.W % 2 == 0 && tc.tw < g.W) tc.tw += tc.tw & 1;  // even widths keep cell pairs inside tiles
   const int cap = (dir ? kAdjMaxK : kFwdMaxK) * kThreads;
+  if (std::min(tc.tw, g.nx) > kThreads) {  // one thread per receiver column of a tile
+    snprintf(err, errn, "sweep: tile width %d records more than %d receiver columns", tc.tw, kThreads);
+    return SS_EUNSUP;
+  }
   if (ext_cells(g.H, g.W, tc) > cap ||
       lds_floats(ext_cells(g.H, g.W, tc), ext_width(g.W, tc), g.pad, dir, nullptr, nullptr, nullptr) >
           kMaxLdsFloats) {
