@@ -9,7 +9,10 @@ archive's exact-adjoint FWI, archive/toy/slopes/fwi.py:138-164):
   [2, 10] cells; input = (shot-summed misfit gradient, shot-summed
   illumination) of the current model against the true gathers, each
   normalised by its maximum; target = (true - current) / 1000 (km/s);
-* network: ``PrecondCNN`` trained with Adam on the hand-written kernels;
+* network: ``PrecondCNN`` trained with Adam on the hand-written kernels, then
+  dataset aggregation: pairs along the trained network's own refinement
+  trajectories are added and a fresh network is trained on all of them (twice
+  by default), so it learns the models its own updates produce;
 * refinement: start = smooth(true, 8); each iteration computes the gradient
   (HIP forward + exact adjoint) and
     plain:   d = -g/max|g|, backtracking on the data misfit from a 100 m/s
@@ -158,6 +161,36 @@ def make_pairs(sim, n_pairs, seed=0, batch=128, log=None):
     return torch.cat(xs), torch.cat(ts)
 
 
+def trajectory_pairs(sim, net, n_models, steps, seed=0, batch=128, log=None):
+    """Training pairs along the learned refinement's own trajectories: for seeded
+    true models and smooth starts (sigma in [2, 10] cells), the features and target
+    (true - current) / 1000 at the start and after each of ``steps`` learned steps
+    (with their fallback).  Adding these to the training set and training again
+    (dataset aggregation) teaches the network the models its own updates produce,
+    not only smooth starts."""
+    rng = np.random.default_rng(seed)
+    xs, ts = [], []
+    made = 0
+    while made < n_models:
+        b = min(batch, n_models - made)
+        true = synth.mixed(b, seed=int(rng.integers(1 << 30)), nz=sim.geom.nz, nx=sim.geom.nx)
+        sig = rng.uniform(2.0, 10.0, b)
+        cur = np.stack([synth.smooth(true[k:k + 1], sig[k])[0] for k in range(b)])
+        prob = Problem(sim, true)
+        tt = torch.tensor(true, dtype=torch.float32, device=sim.device)
+        v = torch.tensor(cur, dtype=torch.float32, device=sim.device)
+        for k in range(steps + 1):
+            x, f = prob.features(v)
+            xs.append(x)
+            ts.append(((tt - v) / 1000.0)[:, None].contiguous())
+            if k < steps:
+                v, _, _ = learned_step(prob, net, v, x, f)
+        made += b
+        if log:
+            log(f'  trajectory models {made}/{n_models}')
+    return torch.cat(xs), torch.cat(ts)
+
+
 def train(net, x, t, epochs, batch=64, lr=1e-3, seed=0, log=None):
     n = len(x)
     g = torch.Generator().manual_seed(seed)
@@ -185,7 +218,11 @@ def iterations_to_reach(hist, target):
     return int(hit[0]) if len(hit) else None
 
 
-def run(pairs=20000, epochs=8, n_eval=256, iters=10, seed=0, log=print):
+def run(pairs=20000, epochs=8, n_eval=256, iters=10, seed=0, dagger_rounds=2, traj_steps=3, log=print):
+    """Generate pairs, train, then ``dagger_rounds`` times add ``pairs`` more pairs
+    from the current network's own refinement trajectories (pairs / (traj_steps + 1)
+    models, traj_steps learned steps each) and train a fresh network on everything;
+    evaluate the last network against the plain step on held-out models."""
     sim = gpu.Sim(device='cuda')
     t0 = time.time()
     log(f'[refine] generating {pairs} training pairs')
@@ -196,6 +233,18 @@ def run(pairs=20000, epochs=8, n_eval=256, iters=10, seed=0, log=print):
     t1 = time.time()
     losses = train(net, x, t, epochs, log=log)
     t_train = time.time() - t1
+    for r in range(dagger_rounds):
+        t2 = time.time()
+        log(f'[refine] aggregation round {r + 1}: trajectories of the current network')
+        xa, ta = trajectory_pairs(sim, net, max(1, pairs // (traj_steps + 1)), traj_steps, seed=seed + 1 + r,
+                                  log=log)
+        x, t = torch.cat([x, xa]), torch.cat([t, ta])
+        t_pairs += time.time() - t2
+        t3 = time.time()
+        net = PrecondCNN('cuda', seed=seed)
+        losses = train(net, x, t, epochs, log=log)
+        t_train += time.time() - t3
+    n_train = len(x)
     del x, t
     true = synth.mixed(n_eval, seed=10_000 + seed)
     start = synth.smooth(true, 8.0)
@@ -221,7 +270,8 @@ def run(pairs=20000, epochs=8, n_eval=256, iters=10, seed=0, log=print):
         return dict(zip(('mae', 'rmse', 'ssim'), m))
 
     result = {
-        'pairs': pairs, 'epochs': epochs, 'eval_models': n_eval, 'iters': iters, 'seed': seed,
+        'pairs': pairs, 'training_pairs': n_train, 'dagger_rounds': dagger_rounds, 'traj_steps': traj_steps,
+        'epochs': epochs, 'eval_models': n_eval, 'iters': iters, 'seed': seed,
         'train_loss': losses, 'time_pairs_s': t_pairs, 'time_train_s': t_train,
         'time_refine_plain_s': t_plain, 'time_refine_learned_s': t_learned,
         'refine_iterations_per_s': n_eval * iters / t_plain,
@@ -245,9 +295,11 @@ def main():
     ap.add_argument('--eval', type=int, default=256)
     ap.add_argument('--iters', type=int, default=10)
     ap.add_argument('--seed', type=int, default=0)
+    ap.add_argument('--dagger-rounds', type=int, default=2)
+    ap.add_argument('--traj-steps', type=int, default=3)
     ap.add_argument('--out', default=None)
     args = ap.parse_args()
-    result, net = run(args.pairs, args.epochs, args.eval, args.iters, args.seed,
+    result, net = run(args.pairs, args.epochs, args.eval, args.iters, args.seed, args.dagger_rounds, args.traj_steps,
                       log=lambda m: print(m, file=sys.stderr, flush=True))
     text = json.dumps(result)
     print(text, flush=True)
