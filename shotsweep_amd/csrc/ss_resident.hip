@@ -195,7 +195,8 @@ size_t adj_lds_bytes(const Geom &g) {
   return fwd_lds_bytes(g) + (size_t)g.nz * g.nx * sizeof(float);
 }
 
-// Paired kernels (even plane width): [pair table | illumination (adjoint) | guard | plane | guard].
+// Paired kernels (even plane width): [pair table | illumination (adjoint) | guard | plane | guard],
+// and for the adjoint after them its history ring and residual row (kRingBytes).
 // The illumination sits right after the table, so its byte offsets (bits 16-30 of a
 // model pair's descriptor) are relative to an address the LDS instruction adds as
 // an immediate.
@@ -204,8 +205,17 @@ constexpr int pair_table_floats(int pad) { return (pad + 1) * (pad + 1) * 4; }
 size_t fwd_pair_lds_bytes(const Geom &g) {
   return (size_t)(pair_table_floats(g.pad) + g.H * g.W + 2 * pair_guard(g.W)) * sizeof(float);
 }
+// Barrier for loops whose cross-wave traffic is LDS only: this wave's LDS
+// accesses complete, then s_barrier.  Unlike __syncthreads() it does not wait
+// for outstanding buffer loads, including direct-to-LDS loads whose data only
+// the issuing wave reads.
+__device__ __forceinline__ void lds_barrier() {
+  asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+constexpr int kRingBytes = 3 * 8 * 1024 + 288;  // history ring + residual row of the paired adjoint
 size_t adj_pair_lds_bytes(const Geom &g) {
-  return (size_t)(pair_table_floats(g.pad) + g.nz * g.nx + g.H * g.W + 2 * pair_guard(g.W)) * sizeof(float);
+  return (size_t)(pair_table_floats(g.pad) + g.nz * g.nx + g.H * g.W + 2 * pair_guard(g.W)) * sizeof(float) +
+         kRingBytes;
 }
 
 // NZ, NX, PAD > 0 and ROW >= 0 fix the geometry at compile time (0 / -1: runtime).
@@ -635,8 +645,12 @@ __global__ __launch_bounds__(kAdjThreads) void adj_resident_kernel(Geom g, int t
 // Paired adjoint for planes of even width (the 150 x 150 plane): thread t owns
 // the cell pairs (2q, 2q+1), q = t + 512 j.  Seven aligned 8-byte LDS reads per
 // pair give both cells' lap_adjoint taps and one 16-byte read their sponge
-// coefficients; the Laplacian history, residual and c2 move two cells per
-// buffer instruction; the illumination of a model pair is one 8-byte
+// coefficients; the Laplacian history arrives four pairs ahead through a ring
+// of direct-to-LDS loads (the waits for it counted by hand, since the compiler
+// tracks no data that such a load writes), the residual row one step ahead the
+// same way, and c2 two cells per buffer instruction two pairs ahead into
+// registers; within the step loop the barriers wait for LDS accesses only, so
+// ring loads stay in flight across them; the illumination of a model pair is one 8-byte
 // read-modify-write at the address its descriptor holds.  Per cell the
 // arithmetic and its order are those of adj_resident_kernel.  Interior, model
 // and receiver-row ranges all start on even columns, so both cells of a pair
@@ -646,9 +660,6 @@ __global__ __launch_bounds__(kAdjThreads) void adj_resident_kernel(Geom g, int t
 // rim, made by c2a_kernel), the first product of q = ((c2 A) m1) / 12, so the
 // q update needs neither the sponge table nor the interior flag.  C2A = false:
 // `c2` is c2 itself and A comes from the table.
-// history loads are issued this many pairs ahead (a divisor of the pair count)
-template <int KP>
-constexpr int pair_prefetch() { return KP % 2 == 0 ? 2 : (KP % 3 == 0 ? 3 : 1); }
 template <int KP, int NZ, int NX, int PAD, int ROW, bool C2A>
 __global__ __launch_bounds__(kAdjThreads) void adj_pair_kernel(Geom g, int t0, int t1, const float *__restrict__ c2,
                                                                const float *__restrict__ A,
@@ -730,20 +741,50 @@ __global__ __launch_bounds__(kAdjThreads) void adj_pair_kernel(Geom g, int t0, i
   const __amdgpu_buffer_rsrc_t rrs = make_rsrc(res + prop * g.nt * nx, (int64_t)g.nt * nx);
   const __amdgpu_buffer_rsrc_t hrs = make_rsrc(hist + prop * nslots * HW, (int64_t)nslots * HW);
   __syncthreads();
-  constexpr int D = pair_prefetch<KP>();       // history loads D pairs ahead
+  // History ring in LDS after Q's upper guard: three double slots of 8 KB, each
+  // holding two consecutive pairs of every thread (per wave 2 x 64 x 8 B).  One
+  // 16-byte direct-to-LDS buffer load per wave moves two pairs (lanes 0-31 the
+  // first, 32-63 the second); pairs are loaded four ahead.  Double pair n (counted
+  // across steps) lives in slot n mod 3, so slot bases rotate by two every step.
+  constexpr uint32_t kRingAddr = (uint32_t)(tabf + nzx + 2 * guard + HW) * 4;
+  static_assert(KP % 2 == 0 && (KP / 2) % 3 == 2, "two pairs per load; slots rotate by two per step");
+  const uint32_t wv = __builtin_amdgcn_readfirstlane((uint32_t)t >> 6);
+  const uint32_t lane = (uint32_t)t & 63u;
+  const uint32_t dma_off = 8u * (wv * 64u) + (lane >> 5) * (8u * kAdjThreads) + (lane & 31u) * 16u;
+  // T_i = slot base + wv * 512: a thread reads its pair at o8 + T_i (+ 512 for the second pair)
+  // (o8 = 8 t = wv * 512 + 8 lane); the wave's DMA target is T_i + wv * 512
+  uint32_t T0 = kRingAddr + wv * 512u, T1 = T0 + 8192u, T2 = T0 + 16384u;
+  auto ring = [&](int i) -> uint32_t { return i == 0 ? T0 : (i == 1 ? T1 : T2); };
+  auto issue = [&](int jn, int s) {  // pairs jn, jn+1 (jn >= KP: of the next step) of time slot s
+    const int p = jn < KP ? jn : jn - KP;
+    __attribute__((address_space(3))) void *dst =
+        (__attribute__((address_space(3))) void *)(uintptr_t)(ring((jn / 2) % 3) + wv * 512u);
+    __builtin_amdgcn_raw_ptr_buffer_load_lds(hrs, dst, 16, dma_off + (uint32_t)p * 8u * kAdjThreads,
+                                             (uint32_t)(s * HW * 4), 0, 0);
+  };
+  // residual row of the next step: 18 lanes of wave 0 move 72 floats into LDS after the ring
+  constexpr uint32_t kResAddr = kRingAddr + 3u * 8192u;
+  auto issue_res = [&](int it_next) {
+    if (wv == 0 && lane < 18u)
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(rrs, (__attribute__((address_space(3))) void *)(uintptr_t)kResAddr, 16,
+                                               lane * 16u, (uint32_t)(it_next * nx * 4), 0, 0);
+  };
+  constexpr int jrec = (rec0 / 2) / kAdjThreads;
+  static_assert((rec1 / 2 - 1) / kAdjThreads == jrec && jrec >= 4, "receiver pairs in one pair slot, after the first");
+  static_assert(nx % 2 == 0 && nx <= 72, "the residual row fits the staging slot");
+  static_assert(rec0 / 2 >= jrec * kAdjThreads && rec1 / 2 - 1 < jrec * kAdjThreads + 64,
+                "receiver pairs belong to wave 0, which stages the residual");
+  issue_res(t1 - 1);
   constexpr int DC = kAdjCPrefetch;            // c2 (or c2 A) loads DC pairs ahead
-  static_assert(KP % D == 0 && KP % DC == 0, "pair count must be a multiple of the prefetch distances");
-  float2 Lb[D], Cb[DC];  // history and c2 (or c2 A) of the pairs ahead
-#pragma unroll
-  for (int d = 0; d < D; ++d) {
-    const bool live = t + d * kAdjThreads < NP;
-    Lb[d] = live ? buf_load2(hrs, t * 8, ((t1 - 1 - t0) * HW + 2 * d * kAdjThreads) * 4) : make_float2(0.0f, 0.0f);
-  }
+  static_assert(KP % DC == 0, "pair count must be a multiple of the prefetch distance");
+  float2 Cb[DC];
 #pragma unroll
   for (int d = 0; d < DC; ++d) {
     const bool live = t + d * kAdjThreads < NP;
     Cb[d] = live ? buf_load2(crs, t * 8, d * kAdjThreads * 8) : make_float2(0.0f, 0.0f);
   }
+  issue(0, t1 - 1 - t0);
+  issue(2, t1 - 1 - t0);
   for (int it = t1 - 1; it >= t0; --it) {
     const int slot = it - t0;
     const float wt = w[it];
@@ -752,16 +793,12 @@ __global__ __launch_bounds__(kAdjThreads) void adj_pair_kernel(Geom g, int t0, i
 #pragma unroll
     for (int j = 0; j < KP; ++j) {
       asm volatile("" : "+v"(o8), "+v"(pk[j]));  // one pair at a time
-      const float2 L = Lb[j % D];
-      {
-        const int jn = j + D;
-        if (jn < KP) {
-          if (t + jn * kAdjThreads < NP)
-            Lb[j % D] = buf_load2(hrs, o8, (slot * HW + 2 * jn * kAdjThreads) * 4);
-        } else if (it > t0 && t + (jn - KP) * kAdjThreads < NP) {
-          Lb[j % D] = buf_load2(hrs, o8, ((slot - 1) * HW + 2 * (jn - KP) * kAdjThreads) * 4);
-        }
+      if (j % 2 == 0) {
+        issue(j + 4, j + 4 < KP ? slot : (slot > 0 ? slot - 1 : slot));
+        // the load of pairs j, j+1 is complete once at most the two issued after it are outstanding
+        asm volatile("s_waitcnt vmcnt(2)" ::: "memory");
       }
+      const float2 L = lds_ld2(o8 + ring((j / 2) % 3) + (uint32_t)(j & 1) * 512u);
       if (t + j * kAdjThreads < NP) {
         const int q = (int)(o8 >> 3) + j * kAdjThreads;
         // One base address per pair (two rows up); the seven taps are
@@ -780,12 +817,11 @@ __global__ __launch_bounds__(kAdjThreads) void adj_pair_kernel(Geom g, int t0, i
         mu1 = mu1 + lt1;
         mu0 = mu0 - e.y * m2[j].x;
         mu1 = mu1 - e.w * m2[j].y;
-        if (j * kAdjThreads < rec1 / 2 && (j + 1) * kAdjThreads > rec0 / 2) {
-          if (q >= rec0 / 2 && q < rec1 / 2) {
-            const float2 rv = buf_load2(rrs, (2 * q - rec0) * 4, it * nx * 4);
-            mu0 = mu0 + rv.x;
-            mu1 = mu1 + rv.y;
-          }
+        if (j == jrec && q >= rec0 / 2 && q < rec1 / 2) {
+          // staged by the previous step; complete since the ring loads issued after it are
+          const float2 rv = lds_ld2(kResAddr + (uint32_t)(2 * q - rec0) * 4u);
+          mu0 = mu0 + rv.x;
+          mu1 = mu1 + rv.y;
         }
         ga[j].x = ga[j].x + ((0.5f * e.x) * mu0) * L.x;  // A = 0.5 (2A), exact
         ga[j].y = ga[j].y + ((0.5f * e.z) * mu1) * L.y;
@@ -804,9 +840,12 @@ __global__ __launch_bounds__(kAdjThreads) void adj_pair_kernel(Geom g, int t0, i
         asm volatile("" : "+v"(m1[j].x), "+v"(m1[j].y), "+v"(m2[j].x), "+v"(m2[j].y), "+v"(ga[j].x),
                      "+v"(ga[j].y));
       }
+      // the ring slot just read is refilled two pairs later: its read must be complete
+      asm volatile("" ::"v"(L.x), "v"(L.y) : "memory");
     }
-    __syncthreads();
+    lds_barrier();
     int off = t;
+    if (it > t0) issue_res(it - 1);
     asm volatile("" : "+v"(off));  // keep per-cell descriptors from being hoisted out of the time loop
     opaque(pk);
 #pragma unroll
@@ -829,8 +868,15 @@ __global__ __launch_bounds__(kAdjThreads) void adj_pair_kernel(Geom g, int t0, i
         }
       }
     }
-    __syncthreads();
+    lds_barrier();
+    {
+      const uint32_t x = T2;  // next step: double pair n + 11 is in slot (n + 2) mod 3
+      T2 = T1;
+      T1 = T0;
+      T0 = x;
+    }
   }
+  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // the ring loads of the last step land before exit
 #pragma unroll
   for (int j = 0; j < KP; ++j) {
     if (t + j * kAdjThreads < NP) {
