@@ -6,10 +6,17 @@
 // Convolution forward/backward-data: one block computes a 16 x 16 pixel tile
 // of one image for a group of 8 output channels; the input tile (with a
 // one-pixel halo) for all input channels and the group's weights are staged
-// in LDS, each thread accumulates 8 outputs of one pixel.  Backward-weight:
-// one block per (output channel, input channel) reduces the 9 tap products
-// and (for ci = 0) the bias over the whole batch in a fixed order, so results
-// are deterministic.
+// in LDS, each thread accumulates 8 outputs of one pixel.  Backward-weight
+// (at most 32 channels each way): a reduction over batch x pixels of
+// dz[co] x[ci] shifted by each tap, i.e. nine 32 x 32 products with a long K,
+// on the matrix cores (v_mfma_f32_32x32x2f32, FP32 in and out): a workgroup
+// stages a two-row band of one image (x with a one-pixel halo, dz) in LDS, each
+// wave feeds pixel pairs to nine per-tap accumulators, the four waves of a
+// workgroup are summed in a fixed order and the per-workgroup partials by a
+// second kernel in a fixed order, so results are deterministic.  Wider layers
+// run the VALU kernel (one block per channel pair).
+#include <algorithm>
+
 #include "ss_kernels.h"
 
 namespace ss {
@@ -117,6 +124,133 @@ __global__ __launch_bounds__(256) void conv3x3_wgrad_kernel(int N, int Cin, int 
   }
   if (tid < 9) dw[((int64_t)co * Cin + ci) * 9 + tid] = red[tid][0];
   if (tid == 9 && ci == 0 && db) db[co] = red[9][0];
+}
+
+// Backward-weight on the matrix cores.  Per 32 x 32 x 2 MFMA, lane l gives
+// A[i][k] = dz[co = l % 32][pixel 2 pp + l / 32] and B[k][j] = x[ci = l % 32]
+// at that pixel shifted by the tap, so D[co][ci] accumulates the tap's sum; D
+// element (i, j) sits in register r = 4 (i / 8) + i % 4 of lane j + 32 ((i / 4) % 2).
+constexpr int kWgRows = 2;      // image rows per staged band
+constexpr int kWgThreads = 256; // four waves
+constexpr int kWgAcc = 9 * 16 * 64;      // accumulator slots of a workgroup partial (tap, register, lane)
+constexpr int kWgPart = kWgAcc + 64;     // + per-lane bias partials
+static_assert(kWgAcc % 64 == 0 && kWgPart % 64 == 0, "the finishing blocks cover 64 slots each");
+typedef float v16f __attribute__((ext_vector_type(16)));
+
+__host__ __device__ constexpr int wg_xs(int W) { return ((kWgRows + 2) * (W + 2)) | 1; }  // odd: no bank conflicts
+__host__ __device__ constexpr int wg_ds(int W) { return (kWgRows * W + 1) | 1; }           // + the pad pixel
+// the band tiles, also reused for the wave sums of the partial
+size_t wg_lds_bytes(int W) {
+  return std::max<size_t>((size_t)32 * (wg_xs(W) + wg_ds(W)), (size_t)kWgPart) * sizeof(float);
+}
+
+__global__ __launch_bounds__(kWgThreads) __attribute__((amdgpu_waves_per_eu(2))) void conv3x3_wgrad_mfma_kernel(
+    int N, int Cin, int Cout, int H, int W, const float *__restrict__ x, const float *__restrict__ dz,
+    float *__restrict__ part) {
+  extern __shared__ float sm[];
+  const int XS = wg_xs(W), DS = wg_ds(W);
+  float *xs = sm;            // [ci][kWgRows + 2][W + 2], channels >= Cin zero
+  float *ds = sm + 32 * XS;  // [co][kWgRows * W + pad], channels >= Cout zero
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int i32 = lane & 31, h = lane >> 5;
+  constexpr int kWaves = kWgThreads / 64;
+  const int bands = (H + kWgRows - 1) / kWgRows, nchunks = N * bands;
+  const int RW = kWgRows * W, npairs = (RW + 1) / 2;
+  v16f acc[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) acc[k] = (v16f)(0.0f);
+  float dbp = 0.0f;
+  for (int c = blockIdx.x; c < nchunks; c += gridDim.x) {
+    const int n = c / bands, r0 = (c - n * bands) * kWgRows;
+    __syncthreads();  // the previous band's reads are done
+    // one tile row per wave at a time (row indices are wave-uniform), lanes along the row
+    for (int rowi = wave; rowi < 32 * (kWgRows + 2); rowi += kWaves) {
+      const int ci = rowi / (kWgRows + 2), rr = rowi - ci * (kWgRows + 2), yy = r0 - 1 + rr;
+      const bool live = ci < Cin && yy >= 0 && yy < H;
+      const float *src = x + (((int64_t)n * Cin + (live ? ci : 0)) * H + (live ? yy : 0)) * W;
+      for (int cc = lane; cc < W + 2; cc += 64) {
+        const int xx = cc - 1;
+        xs[ci * XS + rr * (W + 2) + cc] = (live && xx >= 0 && xx < W) ? src[xx] : 0.0f;
+      }
+    }
+    for (int rowi = wave; rowi < 32 * kWgRows; rowi += kWaves) {
+      const int co = rowi / kWgRows, rr = rowi - co * kWgRows;
+      const bool live = co < Cout && r0 + rr < H;
+      const float *src = dz + (((int64_t)n * Cout + (live ? co : 0)) * H + (live ? r0 + rr : 0)) * W;
+      for (int cc = lane; cc < W; cc += 64) ds[co * DS + rr * W + cc] = live ? src[cc] : 0.0f;
+    }
+    if (t < 32) ds[t * DS + RW] = 0.0f;  // the pad pixel of an odd band
+    __syncthreads();
+    // this lane's pixel k = 2 pp + h, advanced by 2 * kWaves per pair
+    int k = 2 * wave + h;
+    int row = k / W, col = k - row * W;
+    for (int pp = wave; pp < npairs; pp += kWaves) {
+      const int kr = k < RW ? row * (W + 2) + col : (kWgRows - 1) * (W + 2) + W - 1;  // pad: any real pixel
+      const float a = ds[i32 * DS + k];
+      dbp += a;
+      const float *xb = xs + i32 * XS + kr;
+      float b[9];
+#pragma unroll
+      for (int tap = 0; tap < 9; ++tap) b[tap] = xb[(tap / 3) * (W + 2) + tap % 3];
+#pragma unroll
+      for (int tap = 0; tap < 9; ++tap) acc[tap] = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b[tap], acc[tap], 0, 0, 0);
+      k += 2 * kWaves;
+      col += 2 * kWaves;
+      while (col >= W) {
+        col -= W;
+        ++row;
+      }
+    }
+  }
+  // waves summed in a fixed order: ((w0 + w1) + w2) + w3
+  float *red = sm;
+  for (int w = 0; w < kWaves; ++w) {
+    __syncthreads();
+    if (wave == w) {
+#pragma unroll
+      for (int tap = 0; tap < 9; ++tap)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          float &o = red[(tap * 16 + r) * 64 + lane];
+          o = w ? o + acc[tap][r] : acc[tap][r];
+        }
+      float &o = red[kWgAcc + lane];
+      o = w ? o + dbp : dbp;
+    }
+  }
+  __syncthreads();
+  for (int i = t; i < kWgPart; i += kWgThreads) part[(int64_t)blockIdx.x * kWgPart + i] = red[i];
+}
+
+// dw, db from the workgroup partials: four threads per slot sum consecutive
+// quarters of the workgroups, then the quarters are added in order.
+__global__ __launch_bounds__(256) void conv3x3_wgrad_finish_kernel(int G, int Cin, int Cout,
+                                                                   const float *__restrict__ part,
+                                                                   float *__restrict__ dw, float *__restrict__ db) {
+  __shared__ float q4[4][64];
+  const int slot = blockIdx.x * 64 + (threadIdx.x & 63), quarter = threadIdx.x >> 6;
+  const int g0 = (int)((int64_t)G * quarter / 4), g1 = (int)((int64_t)G * (quarter + 1) / 4);
+  float s = 0.0f;
+  if (slot < kWgPart)
+    for (int g = g0; g < g1; ++g) s += part[(int64_t)g * kWgPart + slot];
+  q4[quarter][threadIdx.x & 63] = s;
+  __syncthreads();
+  const int l = threadIdx.x & 63;
+  float v = 0.0f;
+  if (quarter == 0 && slot < kWgPart) {
+    v = ((q4[0][l] + q4[1][l]) + q4[2][l]) + q4[3][l];
+    if (slot < kWgAcc) {
+      const int tap = slot / 1024, r = (slot / 64) % 16, ln = slot % 64;
+      const int co = (r / 4) * 8 + (ln / 32) * 4 + r % 4, ci = ln % 32;
+      if (co < Cout && ci < Cin) dw[((int64_t)co * Cin + ci) * 9 + tap] = v;
+    }
+  }
+  if (blockIdx.x * 64 != kWgAcc) return;  // the last block holds the per-lane bias partials
+  __syncthreads();
+  if (quarter == 0) q4[0][l] = v;
+  __syncthreads();
+  // lanes co and co + 32 hold the same output channel
+  if (quarter == 0 && l < 32 && l < Cout && db) db[l] = q4[0][l] + q4[0][l + 32];
 }
 
 __global__ void relu_mask_kernel(int64_t n, const float *__restrict__ dy, const float *__restrict__ y,
@@ -248,10 +382,38 @@ int conv3x3(int N, int Cin, int Cout, int H, int W, const float *x, const float 
   return hip_status(hipGetLastError(), "conv3x3 launch");
 }
 
+static int device_attr(hipDeviceAttribute_t a, int fallback) {
+  int dev = 0, v = 0;
+  if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&v, a, dev) != hipSuccess) return fallback;
+  return v;
+}
+
 int conv3x3_wgrad(int N, int Cin, int Cout, int H, int W, const float *x, const float *dz, float *dw, float *db,
                   hipStream_t st) {
-  conv3x3_wgrad_kernel<<<dim3(Cout, Cin), 256, 0, st>>>(N, Cin, Cout, H, W, x, dz, dw, db);
-  return hip_status(hipGetLastError(), "conv3x3_wgrad launch");
+  const size_t lds = wg_lds_bytes(W);
+  if (N < 1 || Cin > 32 || Cout > 32 || lds > (size_t)device_attr(hipDeviceAttributeMaxSharedMemoryPerBlock, 65536)) {
+    conv3x3_wgrad_kernel<<<dim3(Cout, Cin), 256, 0, st>>>(N, Cin, Cout, H, W, x, dz, dw, db);
+    return hip_status(hipGetLastError(), "conv3x3_wgrad launch");
+  }
+  const int64_t nchunks = (int64_t)N * ((H + kWgRows - 1) / kWgRows);
+  const int G = (int)std::min<int64_t>(nchunks, 2 * (int64_t)device_attr(hipDeviceAttributeMultiprocessorCount, 256));
+  // per-workgroup partials in stream-ordered scratch (freed once the finishing kernel has run)
+  float *part = nullptr;
+  hipError_t e = hipMallocAsync((void **)&part, (size_t)G * kWgPart * sizeof(float), st);
+  if (e != hipSuccess) return hip_status(e, "conv3x3_wgrad scratch");
+  e = hipFuncSetAttribute((const void *)conv3x3_wgrad_mfma_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                          (int)lds);
+  if (e == hipSuccess) {
+    conv3x3_wgrad_mfma_kernel<<<G, kWgThreads, lds, st>>>(N, Cin, Cout, H, W, x, dz, part);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) {
+    conv3x3_wgrad_finish_kernel<<<kWgPart / 64, 256, 0, st>>>(G, Cin, Cout, part, dw, db);
+    e = hipGetLastError();
+  }
+  const hipError_t f = hipFreeAsync(part, st);
+  if (e == hipSuccess) e = f;
+  return hip_status(e, "conv3x3_wgrad launch");
 }
 
 int relu_mask(int64_t n, const float *dy, const float *y, float *dz, hipStream_t st) {

@@ -63,3 +63,37 @@ def test_training_reduces_loss():
         last = net.train_step(x, t, lr=3e-3)
     print(f'  loss {first:.4e} -> {last:.4e}')
     assert last < 0.1 * first
+
+
+@pytest.mark.parametrize('N,Cin,Cout,H,W', [
+    (40, 32, 32, 70, 70),   # 1400 two-row bands: several per workgroup
+    (9, 2, 32, 37, 53),     # first layer, odd sizes (a pad pixel in every band)
+    (5, 32, 1, 70, 70),     # last layer
+    (3, 8, 40, 12, 10),     # more than 32 output channels: the VALU kernel
+])
+def test_weight_gradient_matches_float64_and_is_deterministic(N, Cin, Cout, H, W):
+    """ss_conv3x3_wgrad_f32 against the FP64 weight and bias gradients of a 3x3
+    'same' convolution (torch.nn.grad.conv2d_weight on the CPU); a second call
+    gives the same bits."""
+    case(f'conv3x3 weight gradient N={N} Cin={Cin} Cout={Cout} {H}x{W}')
+    from shotsweep_amd import _lib
+    g = torch.Generator().manual_seed(N * 1000 + Cin)
+    x = torch.randn(N, Cin, H, W, generator=g)
+    dz = torch.randn(N, Cout, H, W, generator=g)
+    want = torch.nn.grad.conv2d_weight(x.double(), (Cout, Cin, 3, 3), dz.double(), padding=1)
+    want_b = dz.double().sum((0, 2, 3))
+    xd, dzd = x.cuda(), dz.cuda()
+    outs = []
+    for _ in range(2):
+        dw = torch.full((Cout, Cin, 3, 3), float('nan'), device='cuda')
+        db = torch.full((Cout,), float('nan'), device='cuda')
+        _lib.check(_lib.load().ss_conv3x3_wgrad_f32(N, Cin, Cout, H, W, _lib.ptr(xd), _lib.ptr(dzd), _lib.ptr(dw),
+                                                   _lib.ptr(db), torch.cuda.current_stream().cuda_stream),
+                   'conv3x3_wgrad')
+        torch.cuda.synchronize()
+        outs.append((dw.cpu(), db.cpu()))
+    ew = float((outs[0][0].double() - want).norm() / want.norm())
+    eb = float((outs[0][1].double() - want_b).norm() / want_b.norm())
+    print(f'  weights rel {ew:.2e}, bias rel {eb:.2e}')
+    assert ew < 1e-5 and eb < 1e-5  # FP32 sums of up to 196k products
+    assert torch.equal(outs[0][0], outs[1][0]) and torch.equal(outs[0][1], outs[1][1])
