@@ -3,10 +3,12 @@
 // loss gradient and Adam.  FP32, NCHW.  No reference counterpart (the
 // reference never implemented test-time refinement, README.md:25).
 //
-// Convolution forward/backward-data: one block computes a 16 x 16 pixel tile
-// of one image for a group of 8 output channels; the input tile (with a
-// one-pixel halo) for all input channels and the group's weights are staged
-// in LDS, each thread accumulates 8 outputs of one pixel.  Backward-weight
+// Convolution forward/backward-data with 16-32 output channels: an implicit
+// GEMM on the matrix cores (conv3x3_mfma_kernel).  With fewer output channels:
+// one block computes a 16 x 16 pixel tile of one image for a group of 8
+// output channels; the input tile (with a one-pixel halo) for all input
+// channels and the group's weights are staged in LDS, each thread accumulates
+// 8 outputs of one pixel (conv3x3_kernel).  Backward-weight
 // (at most 32 channels each way): a reduction over batch x pixels of
 // dz[co] x[ci] shifted by each tap, i.e. nine 32 x 32 products with a long K,
 // on the matrix cores (v_mfma_f32_32x32x2f32, FP32 in and out): a workgroup
@@ -126,6 +128,100 @@ __global__ __launch_bounds__(256) void conv3x3_wgrad_kernel(int N, int Cin, int 
   if (tid == 9 && ci == 0 && db) db[co] = red[9][0];
 }
 
+// Convolution (forward, or with `flip` the input gradient) on the matrix cores:
+// an implicit GEMM D[co][pixel] = sum over (tap, ci) of w x shifted.  A
+// workgroup stages a band of kCvRows image rows of one image (all input
+// channels, one-pixel halo) in LDS; each wave holds the whole weight matrix in
+// registers (A operands: lane l gives w[co = l % 32][ci = 2 m + l / 32][tap]
+// for K step (tap, m)) and runs 32-pixel tiles of the band (B operands: lane l
+// gives x[ci = 2 m + l / 32] at pixel l % 32 of the tile, shifted by the tap).
+// Each wave runs two tiles at a time (two accumulator chains).  The
+// accumulators start at the bias; ReLU on the way out.  Layers with fewer than
+// 16 output channels run the VALU kernel.
+typedef float v16f __attribute__((ext_vector_type(16)));
+constexpr int kCvRows = 6;
+constexpr int kCvThreads = 256;
+__host__ __device__ constexpr int cv_xs(int W) { return ((kCvRows + 2) * (W + 2)) | 1; }
+size_t cv_lds_bytes(int CINP, int W) { return (size_t)2 * CINP * cv_xs(W) * sizeof(float); }
+
+template <int CINP>  // input-channel pairs (Cin <= 2 CINP)
+__global__ __launch_bounds__(kCvThreads) __attribute__((amdgpu_waves_per_eu(CINP == 1 ? 2 : 1))) void conv3x3_mfma_kernel(
+    int N, int Cin, int Cout, int H, int W, const float *__restrict__ x, const float *__restrict__ w,
+    const float *__restrict__ b, float *__restrict__ y, int relu, int flip) {
+  extern __shared__ float sm[];
+  const int XS = cv_xs(W);
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int j32 = lane & 31, h = lane >> 5;
+  constexpr int kWaves = kCvThreads / 64;
+  float A[9 * CINP];
+#pragma unroll
+  for (int tap = 0; tap < 9; ++tap)
+#pragma unroll
+    for (int m = 0; m < CINP; ++m) {
+      const int co = j32, ci = 2 * m + h;
+      float v = 0.0f;
+      if (co < Cout && ci < Cin) v = flip ? w[((int64_t)ci * Cout + co) * 9 + (8 - tap)] : w[((int64_t)co * Cin + ci) * 9 + tap];
+      A[tap * CINP + m] = v;
+    }
+  const int bands = (H + kCvRows - 1) / kCvRows, nchunks = N * bands;
+  for (int c = blockIdx.x; c < nchunks; c += gridDim.x) {
+    const int n = c / bands, r0 = (c - n * bands) * kCvRows;
+    const int rows = min(kCvRows, H - r0), npix = rows * W;
+    __syncthreads();  // the previous band's reads are done
+    for (int rowi = wave; rowi < 2 * CINP * (kCvRows + 2); rowi += kWaves) {
+      const int ci = rowi / (kCvRows + 2), rr = rowi - ci * (kCvRows + 2), yy = r0 - 1 + rr;
+      const bool live = ci < Cin && yy >= 0 && yy < H;
+      const float *src = x + (((int64_t)n * Cin + (live ? ci : 0)) * H + (live ? yy : 0)) * W;
+      for (int cc = lane; cc < W + 2; cc += 64) {
+        const int xx = cc - 1;
+        sm[ci * XS + rr * (W + 2) + cc] = (live && xx >= 0 && xx < W) ? src[xx] : 0.0f;
+      }
+    }
+    __syncthreads();
+    // two tiles per pass (two independent accumulator chains)
+    for (int tile = 2 * wave; tile * 32 < npix; tile += 2 * kWaves) {
+      int kp[2];
+      const float *xb[2];
+      v16f acc[2];
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        kp[u] = (tile + u) * 32 + j32;                  // this lane's pixel in the band
+        const int kk = kp[u] < npix ? kp[u] : npix - 1;  // tail lanes read a real pixel and are not stored
+        const int row = kk / W, col = kk - row * W;
+        xb[u] = sm + h * XS + row * (W + 2) + col;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const int co = (r / 4) * 8 + h * 4 + r % 4;
+          acc[u][r] = (b && co < Cout) ? b[co] : 0.0f;
+        }
+      }
+#pragma unroll
+      for (int tap = 0; tap < 9; ++tap)
+#pragma unroll
+        for (int m = 0; m < CINP; ++m) {
+          const int o = 2 * m * XS + (tap / 3) * (W + 2) + tap % 3;
+          const float a = A[tap * CINP + m];
+          acc[0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a, xb[0][o], acc[0], 0, 0, 0);
+          acc[1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a, xb[1][o], acc[1], 0, 0, 0);
+        }
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        if (kp[u] < npix) {
+#pragma unroll
+          for (int r = 0; r < 16; ++r) {
+            const int co = (r / 4) * 8 + h * 4 + r % 4;
+            if (co < Cout) {
+              float v = acc[u][r];
+              if (relu) v = v > 0.0f ? v : 0.0f;
+              y[(((int64_t)n * Cout + co) * H + r0) * W + kp[u]] = v;
+            }
+          }
+        }
+      }
+    }
+  }
+}
+
 // Backward-weight on the matrix cores.  Per 32 x 32 x 2 MFMA, lane l gives
 // A[i][k] = dz[co = l % 32][pixel 2 pp + l / 32] and B[k][j] = x[ci = l % 32]
 // at that pixel shifted by the tap, so D[co][ci] accumulates the tap's sum; D
@@ -135,7 +231,6 @@ constexpr int kWgThreads = 256; // four waves
 constexpr int kWgAcc = 9 * 16 * 64;      // accumulator slots of a workgroup partial (tap, register, lane)
 constexpr int kWgPart = kWgAcc + 64;     // + per-lane bias partials
 static_assert(kWgAcc % 64 == 0 && kWgPart % 64 == 0, "the finishing blocks cover 64 slots each");
-typedef float v16f __attribute__((ext_vector_type(16)));
 
 __host__ __device__ constexpr int wg_xs(int W) { return ((kWgRows + 2) * (W + 2)) | 1; }  // odd: no bank conflicts
 __host__ __device__ constexpr int wg_ds(int W) { return (kWgRows * W + 1) | 1; }           // + the pad pixel
@@ -370,11 +465,28 @@ unsigned grid1(int64_t n) {
 
 }  // namespace
 
+static int device_attr(hipDeviceAttribute_t a, int fallback);
+
 int conv3x3(int N, int Cin, int Cout, int H, int W, const float *x, const float *w, const float *b, float *y,
             int relu, int flip, hipStream_t st) {
   if (Cin > kMaxCin) {
     set_error("conv3x3: at most 32 input channels");
     return SS_EINVAL;
+  }
+  const int CINP = Cin <= 2 ? 1 : 16;
+  const size_t lds = cv_lds_bytes(CINP, W);
+  if (N >= 1 && Cout >= 16 && Cout <= 32 &&
+      lds <= (size_t)device_attr(hipDeviceAttributeMaxSharedMemoryPerBlock, 65536)) {
+    auto kern = CINP == 1 ? conv3x3_mfma_kernel<1> : conv3x3_mfma_kernel<16>;
+    hipError_t e = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return hip_status(e, "conv3x3 attribute");
+    const int64_t nchunks = (int64_t)N * ((H + kCvRows - 1) / kCvRows);
+    // one resident workgroup per CU for the wide kernel (its registers allow one wave per SIMD), two otherwise;
+    // each loads the weights once and walks bands
+    const int per_cu = CINP == 1 ? 2 : 1;
+    const int G = (int)std::min<int64_t>(nchunks, per_cu * (int64_t)device_attr(hipDeviceAttributeMultiprocessorCount, 256));
+    kern<<<G, kCvThreads, lds, st>>>(N, Cin, Cout, H, W, x, w, b, y, relu, flip);
+    return hip_status(hipGetLastError(), "conv3x3 launch");
   }
   const int tiles = ((H + kTile - 1) / kTile) * ((W + kTile - 1) / kTile);
   const dim3 grid(tiles, (Cout + kCoGroup - 1) / kCoGroup, N);
