@@ -44,7 +44,8 @@ extern "C" {
 
 /* Implementation selector for the time-loop entry points. */
 #define SS_IMPL_AUTO 0      /* resident when the plane fits the LDS, else stepwise (ss_rtm_*: tiled
-                               checkpoint segments for batches of at most CUs/2 propagations) */
+                               checkpoint segments for batches of at most CUs/4 propagations or
+                               planes that do not fit the LDS) */
 #define SS_IMPL_STEPWISE 1  /* one launch per time step, planes in HBM (FP32/FP64, any size) */
 #define SS_IMPL_RESIDENT 2  /* one persistent launch per step range, plane in LDS (FP32) */
 #define SS_IMPL_TILED 3     /* ss_rtm_* checkpoint segments on the tiled sweep kernels (FP32) */

@@ -14,9 +14,10 @@
 // library-owned side stream while segment k's adjoint runs on the caller's
 // stream (two segment history buffers alternate), so the recompute fills CUs
 // the adjoint leaves idle.  Each launch computes exactly what it computes in
-// sequence, so results do not depend on the overlap.  Small batches (at most
-// half as many propagations as CUs) run their segments on the tiled sweep
-// kernels, every propagation split into tiles over all CUs.
+// sequence, so results do not depend on the overlap.  Small batches (at most a
+// quarter as many propagations as CUs), and planes too large for the LDS, run
+// their segments on the tiled sweep kernels, every propagation split into tiles
+// over all CUs.
 #include <algorithm>
 #include <cstring>
 #include <map>
@@ -88,8 +89,9 @@ int device_cus() {
 
 // Time-loop implementation of a call.  Whole history: the resident kernels when the
 // plane fits the LDS (FP32), else the stepwise ones.  Checkpoint segments: also the
-// tiled kernels, chosen by 'auto' for batches of at most half as many propagations
-// as the GPU has CUs (one workgroup per propagation would leave most CUs idle).
+// tiled kernels, chosen by 'auto' for batches of at most a quarter as many
+// propagations as the GPU has CUs (one workgroup per propagation would leave most
+// CUs idle) and for planes that do not fit the LDS.
 template <typename T>
 int choose_impl(const Geom &g, int segment, int req, Impl *out) {
   constexpr bool f32 = std::is_same<T, float>::value;
@@ -115,7 +117,11 @@ int choose_impl(const Geom &g, int segment, int req, Impl *out) {
       return SS_OK;
     case SS_IMPL_AUTO:
     case SS_IMPL_UNTILED:
-      if (req == SS_IMPL_AUTO && tiled_ok && 2 * (int64_t)g.nb * g.ns <= device_cus())
+      // resident segments cost about one resident round per segment pair whatever the batch (the
+      // recompute runs on CUs the adjoint leaves idle); tiled segments scale with the batch and
+      // win below about a quarter of the CUs (80 propagations on 256 CUs: 12.6 vs 13.4 ms per
+      // gradient).  Planes that do not fit the LDS take the tiled kernels at any batch size.
+      if (req == SS_IMPL_AUTO && tiled_ok && (!res_ok || 4 * (int64_t)g.nb * g.ns <= device_cus()))
         *out = kTiled;
       else
         *out = res_ok ? kResident : kStepwise;
