@@ -19,7 +19,7 @@ def test_refinement_end_to_end_small():
 
 
 def test_learned_preconditioner_gain():
-    """What the trained network gains (reduced and seeded): 2,000
+    """What the trained network gains (the learned-preconditioner workload, reduced and seeded): 2,000
     training pairs plus two rounds of pairs from the network's own refinement
     trajectories, 32 held-out models, 10 iterations.  The learned step reaches the
     plain adjoint-gradient step's 10-iteration misfit in at most half the
