@@ -504,7 +504,11 @@ class Sim:
     def images_device(self, prep, observed, obs_repeat=1, v_mute=None, t_mute=0.12, eps=1e-4, laplacian=True):
         """Device-resident images in one native call: ``prep`` from ``_prepare``,
         ``observed`` a device tensor (nb/obs_repeat, ns, nt, nx) reused by obs_repeat
-        consecutive rows, ``v_mute`` a device tensor (nb,).  Returns (img, illum)."""
+        consecutive rows, ``v_mute`` a device tensor (nb,).  Returns (img, illum).
+
+        ``observed`` may instead be a pinned host tensor: it is then uploaded on a
+        copy stream while the forward stage runs (the forward does not read the
+        data), and the adjoint stage waits for the upload; the results are the same."""
         if eps <= 0:
             raise ValueError('illumination regularization eps must be positive')
         g = self.geom
@@ -513,9 +517,28 @@ class Sim:
         grad = self._scratch('grad', (nb, g.ns, g.nz, g.nx))
         illum = torch.empty(nb, g.ns, g.nz, g.nx, dtype=self.dtype, device=self.kdev)
         img = torch.empty_like(illum)
-        self._rtm(prep, _lib.RTM_FORWARD | _lib.RTM_ADJOINT, seg, self._workspace(nb, seg), observed=observed,
-                  obs_repeat=obs_repeat, v_mute=v_mute, t_mute=t_mute, grad=grad, illum=illum, img=img, eps=eps,
-                  laplacian=laplacian)
+        ws = self._workspace(nb, seg)
+        args = dict(obs_repeat=obs_repeat, v_mute=v_mute, t_mute=t_mute, grad=grad, illum=illum, img=img, eps=eps,
+                    laplacian=laplacian)
+        if observed.is_cuda:
+            self._rtm(prep, _lib.RTM_FORWARD | _lib.RTM_ADJOINT, seg, ws, observed=observed, **args)
+            return img, illum
+        if not observed.is_pinned() or observed.dtype != self.dtype:
+            raise ValueError('host observations must be a pinned tensor of the working dtype')
+        dev = self._scratch('observed', tuple(observed.shape))
+        cur = torch.cuda.current_stream(self.kdev)
+        if getattr(self, '_copy_stream', None) is None:
+            self._copy_stream = torch.cuda.Stream(self.kdev)
+        cs = self._copy_stream
+        cs.wait_stream(cur)  # the buffer's previous readers on this stream are done
+        with torch.cuda.stream(cs):
+            dev.copy_(observed, non_blocking=True)
+            ready = torch.cuda.Event()
+            ready.record(cs)
+        self._rtm(prep, _lib.RTM_FORWARD, seg, ws)
+        cur.wait_event(ready)
+        dev.record_stream(cur)
+        self._rtm(prep, _lib.RTM_ADJOINT, seg, ws, observed=dev, **args)
         return img, illum
 
     @torch.no_grad()

@@ -399,3 +399,23 @@ def test_wide_shallow_grid_records_every_receiver():
                      segment_impl='tiled').gradient(v, obs, v_mute=np.array([5000.0]), t_mute=0.0)
     np.testing.assert_array_equal(gi, oi)
     assert rel(gg, og) < F32_TOL
+
+
+@pytest.mark.parametrize('history', ['full', 'checkpoint'])
+def test_pinned_host_observations_equal_device_observations(history):
+    """images_device with pinned host observations (uploaded on a copy stream beside
+    the forward stage, the adjoint stage after the upload) gives the same bits as
+    with device observations, for the whole history and checkpoint segments."""
+    case(f'pinned host observations, history {history}')
+    sim = gpu.Sim(device=DEV, history=history)
+    models = synth.mixed(2, seed=31)
+    obs = sim.forward(models)[0].contiguous()
+    bgs = np.repeat(synth.smooth(models, 6.0), 3, axis=0)
+    vm = torch.full((6,), 1900.0, device=DEV)
+    prep = sim._prepare(bgs)
+    img_d, ill_d = (t.cpu() for t in sim.images_device(prep, obs, 3, vm))
+    pinned = obs.cpu().pin_memory()
+    img_h, ill_h = (t.cpu() for t in sim.images_device(prep, pinned, 3, vm))
+    assert torch.equal(img_h, img_d) and torch.equal(ill_h, ill_d)
+    with pytest.raises(ValueError, match='pinned'):
+        sim.images_device(prep, obs.cpu(), 3, vm)
