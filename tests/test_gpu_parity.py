@@ -171,6 +171,20 @@ def test_checkpoint_recompute_equals_full_history(golden):
             np.testing.assert_array_equal(full[1], other[1], err_msg=seg_impl)
 
 
+@pytest.mark.parametrize('interval', [1, 2, 7])
+def test_short_resident_segments_equal_full_history(golden, interval):
+    """Resident checkpoint segments of 1, 2 and 7 steps (the paired adjoint's history
+    ring then wraps into the next segment after every few pairs or at once) give the
+    full-history images and illumination bit for bit."""
+    case(f'resident segments of {interval} steps vs full history')
+    obs = np.repeat(golden['obs_f32'], 3, 0)
+    full = gpu.Sim(device='cuda', history='full').images(golden['backgrounds'], obs, v_mute=golden['v_mute'])
+    ck = gpu.Sim(device='cuda', history='checkpoint', checkpoint_interval=interval, segment_impl='resident').images(
+        golden['backgrounds'], obs, v_mute=golden['v_mute'])
+    np.testing.assert_array_equal(full[0], ck[0])
+    np.testing.assert_array_equal(full[1], ck[1])
+
+
 @pytest.mark.parametrize('nx', [28, 27])
 def test_tiled_segments_equal_resident_segments(nx):
     """Checkpointed gradients on a small grid (paired kernels for an even width, cell
