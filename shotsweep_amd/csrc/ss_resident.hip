@@ -785,7 +785,10 @@ __global__ __launch_bounds__(kAdjThreads) void adj_pair_kernel(Geom g, int t0, i
   }
   issue(0, t1 - 1 - t0);
   issue(2, t1 - 1 - t0);
-  for (int it = t1 - 1; it >= t0; --it) {
+  // The time loop runs two steps per iteration with the roles of the two state arrays
+  // swapped in the second, so the new state is written over the oldest one and no
+  // register moves the state along.
+  auto step = [&](float2(&M1)[KP], float2(&M2)[KP], const int it) __attribute__((always_inline)) {
     const int slot = it - t0;
     const float wt = w[it];
     uint32_t o8 = 8u * t;  // the thread's byte offset within a row of pairs
@@ -811,12 +814,12 @@ __global__ __launch_bounds__(kAdjThreads) void adj_pair_kernel(Geom g, int t0, i
         const float lt0 = lapt_sum(r1.x, u2p.x, up.x, um.x, u2m.x, r2.x, r1.y, r0.y, r0.x);
         const float lt1 = lapt_sum(r1.y, u2p.y, up.y, um.y, u2m.y, r2.y, r2.x, r1.x, r0.y);
         const float4 e = pair_coef(pk[j]);  // (2A0, A0 B0, 2A1, A1 B1)
-        float mu0 = e.x * m1[j].x;
-        float mu1 = e.z * m1[j].y;
+        float mu0 = e.x * M1[j].x;
+        float mu1 = e.z * M1[j].y;
         mu0 = mu0 + lt0;
         mu1 = mu1 + lt1;
-        mu0 = mu0 - e.y * m2[j].x;
-        mu1 = mu1 - e.w * m2[j].y;
+        mu0 = mu0 - e.y * M2[j].x;
+        mu1 = mu1 - e.w * M2[j].y;
         if (j == jrec && q >= rec0 / 2 && q < rec1 / 2) {
           // staged by the previous step; complete since the ring loads issued after it are
           const float2 rv = lds_ld2(kResAddr + (uint32_t)(2 * q - rec0) * 4u);
@@ -835,9 +838,8 @@ __global__ __launch_bounds__(kAdjThreads) void adj_pair_kernel(Geom g, int t0, i
           v.y = v.y + L.y * L.y;
           lds_st2(ia, v);
         }
-        m2[j] = m1[j];
-        m1[j] = make_float2(mu0, mu1);
-        asm volatile("" : "+v"(m1[j].x), "+v"(m1[j].y), "+v"(m2[j].x), "+v"(m2[j].y), "+v"(ga[j].x),
+        M2[j] = make_float2(mu0, mu1);  // the new state; M1 (the old state) becomes the next step's m2
+        asm volatile("" : "+v"(M1[j].x), "+v"(M1[j].y), "+v"(M2[j].x), "+v"(M2[j].y), "+v"(ga[j].x),
                      "+v"(ga[j].y));
       }
       // the ring slot just read is refilled two pairs later: its read must be complete
@@ -859,10 +861,10 @@ __global__ __launch_bounds__(kAdjThreads) void adj_pair_kernel(Geom g, int t0, i
       }
       if (t + j * kAdjThreads < NP) {
         if (C2A) {
-          Q2[off + j * kAdjThreads] = div12x2(cv.x * m1[j].x, cv.y * m1[j].y);
+          Q2[off + j * kAdjThreads] = div12x2(cv.x * M2[j].x, cv.y * M2[j].y);
         } else {
           const float4 e = pair_coef(pk[j]);
-          const float2 qq = div12x2((cv.x * (0.5f * e.x)) * m1[j].x, (cv.y * (0.5f * e.z)) * m1[j].y);
+          const float2 qq = div12x2((cv.x * (0.5f * e.x)) * M2[j].x, (cv.y * (0.5f * e.z)) * M2[j].y);
           // both cells of a pair share the interior flag (bit 0)
           Q2[off + j * kAdjThreads] = make_float2(keep_if<0>(qq.x, pk[j]), keep_if<0>(qq.y, pk[j]));
         }
@@ -875,13 +877,20 @@ __global__ __launch_bounds__(kAdjThreads) void adj_pair_kernel(Geom g, int t0, i
       T1 = T0;
       T0 = x;
     }
+  };
+  int it = t1 - 1;
+  for (; it - 1 >= t0; it -= 2) {
+    step(m1, m2, it);
+    step(m2, m1, it - 1);
   }
+  const bool odd = it >= t0;  // one step left: the current state is then in m2
+  if (odd) step(m1, m2, it);
   asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // the ring loads of the last step land before exit
 #pragma unroll
   for (int j = 0; j < KP; ++j) {
     if (t + j * kAdjThreads < NP) {
-      buf_store2(m1[j], srs1, t * 8, j * kAdjThreads * 8);
-      buf_store2(m2[j], srs2, t * 8, j * kAdjThreads * 8);
+      buf_store2(odd ? m2[j] : m1[j], srs1, t * 8, j * kAdjThreads * 8);
+      buf_store2(odd ? m1[j] : m2[j], srs2, t * 8, j * kAdjThreads * 8);
       buf_store2(ga[j], srsg, t * 8, j * kAdjThreads * 8);
     }
   }
