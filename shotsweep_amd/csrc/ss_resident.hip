@@ -736,6 +736,7 @@ __global__ __launch_bounds__(kAdjThreads) void adj_pair_kernel(Geom g, int t0, i
   const int srcq = srci >> 1;
   const float sdv = sd[prop];
   const bool own_src = (srcq % kAdjThreads) == t;
+  const int jsrc = srcq / kAdjThreads;
   float sg = own_src ? src_grad[prop] : 0.0f;
   const int nslots = t1 - t0;
   const __amdgpu_buffer_rsrc_t rrs = make_rsrc(res + prop * g.nt * nx, (int64_t)g.nt * nx);
@@ -791,6 +792,7 @@ __global__ __launch_bounds__(kAdjThreads) void adj_pair_kernel(Geom g, int t0, i
   auto step = [&](float2(&M1)[KP], float2(&M2)[KP], const int it) __attribute__((always_inline)) {
     const int slot = it - t0;
     const float wt = w[it];
+    float2 msrc = make_float2(0.0f, 0.0f);  // the new state of the source's pair (its owner's lane)
     uint32_t o8 = 8u * t;  // the thread's byte offset within a row of pairs
     opaque(pk);
 #pragma unroll
@@ -828,9 +830,7 @@ __global__ __launch_bounds__(kAdjThreads) void adj_pair_kernel(Geom g, int t0, i
         }
         ga[j].x = ga[j].x + ((0.5f * e.x) * mu0) * L.x;  // A = 0.5 (2A), exact
         ga[j].y = ga[j].y + ((0.5f * e.z) * mu1) * L.y;
-        if (j == srcq / kAdjThreads) {
-          if (q == srcq) sg = sg + (((srci & 1) ? mu1 : mu0) * wt) * sdv;
-        }
+        if (j == jsrc) msrc = make_float2(mu0, mu1);  // the source's pair slot (uniform): two selects
         if ((int32_t)pk[j] < 0) {  // model pair: its illumination address is in the descriptor
           const uint32_t ia = kIlAddr + ((pk[j] >> 16) & kPIlMask);
           float2 v = lds_ld2(ia);
@@ -845,6 +845,7 @@ __global__ __launch_bounds__(kAdjThreads) void adj_pair_kernel(Geom g, int t0, i
       // the ring slot just read is refilled two pairs later: its read must be complete
       asm volatile("" ::"v"(L.x), "v"(L.y) : "memory");
     }
+    if (own_src) sg = sg + (((srci & 1) ? msrc.y : msrc.x) * wt) * sdv;  // gpu.py:182, in step order
     lds_barrier();
     int off = t;
     if (it > t0) issue_res(it - 1);

@@ -485,16 +485,16 @@ __global__ __launch_bounds__(kThreads) void sweep_fwd_pair_kernel(SArgs a) {
         const float2 ab0 = lds_table2(db[j] & kTabOff), ab1 = lds_table2((db[j] >> 16) & kTabOff);
         float q0 = leapfrog(r1.x, pn[j].x, ab0.y, cc[j].x, L0, ab0.x);
         float q1 = leapfrog(r1.y, pn[j].y, ab1.y, cc[j].y, L1, ab1.x);
-        if (j == src_j) {  // uniform: only the source's pair slot carries this code
-          if (t == src_t) {
-            if (src_hi) q1 = q1 + wsrc;
-            else q0 = q0 + wsrc;
-          }
-        }
         buf_store2(make_float2(L0, L1), hrs, (d & kQOwn) ? (int)(d & kOffMask) * 4 : kNoAccess, (it - G.tlo) * HW * 4);
         N2[p] = make_float2(q0, q1);
         pn[j] = r1;
       }
+    }
+    // the source, added after damping (gpu.py:114) by the thread that just wrote the source's pair
+    // (once per step, not a select in every pair slot)
+    if (src_j >= 0 && t == src_t) {
+      float *ps = reinterpret_cast<float *>(N2 + t + src_j * kThreads) + src_hi;
+      *ps = *ps + wsrc;
     }
     __syncthreads();
     float *tmp = Pc;
@@ -605,6 +605,7 @@ __global__ __launch_bounds__(kThreads) void sweep_adj_pair_kernel(SArgs a) {
   for (int st = 0; st < n; ++st) {
     const int it = hi - 1 - st;
     const float wt = G.w[it];
+    float2 msrc = make_float2(0.0f, 0.0f);  // the new state of the source's pair (its owner's lane)
     const float2 *C2 = reinterpret_cast<const float2 *>(Qc);
     float2 *N2 = reinterpret_cast<float2 *>(Qn);
 #pragma unroll
@@ -643,15 +644,14 @@ __global__ __launch_bounds__(kThreads) void sweep_adj_pair_kernel(SArgs a) {
           const float2 iv = il[p];
           il[p] = make_float2(iv.x + L.x * L.x, iv.y + L.y * L.y);
         }
-        if (j == src_j) {  // uniform: only the source's pair slot carries this code
-          if (t == src_t) sg = sg + ((src_hi ? mu1 : mu0) * wt) * sdv;
-        }
+        if (j == src_j) msrc = make_float2(mu0, mu1);  // the source's pair slot (uniform): two selects
         m2[j] = m1[j];
         m1[j] = make_float2(mu0, mu1);
         const float2 qq = div12x2(ca[j].x * mu0, ca[j].y * mu1);
         N2[p] = make_float2(keep_if<17>(qq.x, d), keep_if<17>(qq.y, d));  // interior (kQInt)
       }
     }
+    if (src_j >= 0 && t == src_t) sg = sg + ((src_hi ? msrc.y : msrc.x) * wt) * sdv;  // gpu.py:182, step order
     __syncthreads();
     float *tmp = Qc;
     Qc = Qn;
