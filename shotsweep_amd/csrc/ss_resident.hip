@@ -794,6 +794,21 @@ __global__ __launch_bounds__(kAdjThreads) void adj_pair_kernel(Geom g, int t0, i
     const float wt = w[it];
     float2 msrc = make_float2(0.0f, 0.0f);  // the new state of the source's pair (its owner's lane)
     uint32_t o8 = 8u * t;  // the thread's byte offset within a row of pairs
+    uint32_t rb[3] = {o8 + T0, o8 + T1, o8 + T2};  // this thread's read addresses in the three ring slots
+    float2 tp[2][7];
+    // the seven lap_adjoint taps of pair jj: immediate offsets from one base address two rows up
+    auto load_taps = [&](int jj, float2(&d)[7]) {
+      uint32_t a = o8 + (kQAddr + 8u * (uint32_t)(jj * kAdjThreads - W));
+      asm volatile("" : "+v"(a));
+      d[0] = lds_ld2(a);
+      d[1] = lds_ld2(a + 4 * W);
+      d[2] = lds_ld2(a + 8 * W - 8);
+      d[3] = lds_ld2(a + 8 * W);
+      d[4] = lds_ld2(a + 8 * W + 8);
+      d[5] = lds_ld2(a + 12 * W);
+      d[6] = lds_ld2(a + 16 * W);
+    };
+    load_taps(0, tp[0]);
     opaque(pk);
 #pragma unroll
     for (int j = 0; j < KP; ++j) {
@@ -803,16 +818,13 @@ __global__ __launch_bounds__(kAdjThreads) void adj_pair_kernel(Geom g, int t0, i
         // the load of pairs j, j+1 is complete once at most the two issued after it are outstanding
         asm volatile("s_waitcnt vmcnt(2)" ::: "memory");
       }
-      const float2 L = lds_ld2(o8 + ring((j / 2) % 3) + (uint32_t)(j & 1) * 512u);
+      const float2 L = lds_ld2(rb[(j / 2) % 3] + (uint32_t)(j & 1) * 512u);
+      // the next pair's taps are read before this pair computes (one pair of reads in flight)
+      if (j + 1 < KP && t + (j + 1) * kAdjThreads < NP) load_taps(j + 1, tp[(j + 1) & 1]);
       if (t + j * kAdjThreads < NP) {
         const int q = (int)(o8 >> 3) + j * kAdjThreads;
-        // One base address per pair (two rows up); the seven taps are
-        // immediate offsets from it.
-        uint32_t a = o8 + (kQAddr + 8u * (uint32_t)(j * kAdjThreads - W));
-        asm volatile("" : "+v"(a));
-        const float2 u2m = lds_ld2(a), um = lds_ld2(a + 4 * W), r0 = lds_ld2(a + 8 * W - 8);
-        const float2 r1 = lds_ld2(a + 8 * W), r2 = lds_ld2(a + 8 * W + 8), up = lds_ld2(a + 12 * W);
-        const float2 u2p = lds_ld2(a + 16 * W);
+        const float2 u2m = tp[j & 1][0], um = tp[j & 1][1], r0 = tp[j & 1][2], r1 = tp[j & 1][3];
+        const float2 r2 = tp[j & 1][4], up = tp[j & 1][5], u2p = tp[j & 1][6];
         const float lt0 = lapt_sum(r1.x, u2p.x, up.x, um.x, u2m.x, r2.x, r1.y, r0.y, r0.x);
         const float lt1 = lapt_sum(r1.y, u2p.y, up.y, um.y, u2m.y, r2.y, r2.x, r1.x, r0.y);
         const float4 e = pair_coef(pk[j]);  // (2A0, A0 B0, 2A1, A1 B1)
@@ -839,8 +851,7 @@ __global__ __launch_bounds__(kAdjThreads) void adj_pair_kernel(Geom g, int t0, i
           lds_st2(ia, v);
         }
         M2[j] = make_float2(mu0, mu1);  // the new state; M1 (the old state) becomes the next step's m2
-        asm volatile("" : "+v"(M1[j].x), "+v"(M1[j].y), "+v"(M2[j].x), "+v"(M2[j].y), "+v"(ga[j].x),
-                     "+v"(ga[j].y));
+        asm volatile("" : "+v"(ga[j].x), "+v"(ga[j].y));
       }
       // the ring slot just read is refilled two pairs later: its read must be complete
       asm volatile("" ::"v"(L.x), "v"(L.y) : "memory");
