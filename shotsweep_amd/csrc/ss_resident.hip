@@ -50,19 +50,14 @@ namespace {
 #ifndef SS_PREFETCH
 #define SS_PREFETCH 4
 #endif
-#ifndef SS_CPREFETCH
-#define SS_CPREFETCH 2
-#endif
 constexpr int kFwdThreads = 1024;            // forward: p(t-1), c2 + descriptors = 2.5 registers per cell
 constexpr int kAdjThreads = SS_ADJ_THREADS;  // adjoint: m1, m2, imaging sum = 3 registers per cell
 constexpr int kFwdMaxSlots = 24;
 constexpr int kAdjMaxSlots = 48;
 constexpr int kPrefetch = SS_PREFETCH;       // adjoint: Laplacian loads issued this many cells ahead
-constexpr int kAdjCPrefetch = SS_CPREFETCH;  // paired adjoint: c2 loads issued this many pairs ahead
-// pairs per thread of the paired adjoint on the 150 x 150 plane, rounded up to a multiple
-// of the c2 prefetch distance (pairs beyond the plane are skipped at compile time)
-constexpr int kAdjPairs150 = ((150 * 150 / 2 + kAdjThreads - 1) / kAdjThreads + kAdjCPrefetch - 1) / kAdjCPrefetch *
-                             kAdjCPrefetch;
+// pairs per thread of the paired adjoint on the 150 x 150 plane (pairs beyond the plane are
+// skipped at compile time)
+constexpr int kAdjPairs150 = (150 * 150 / 2 + kAdjThreads - 1) / kAdjThreads;
 // slots of the 150 x 150 plane, rounded up to a multiple of the prefetch distance
 constexpr int kAdj150 = ((150 * 150 + kAdjThreads - 1) / kAdjThreads + kPrefetch - 1) / kPrefetch * kPrefetch;
 
@@ -776,8 +771,9 @@ __global__ __launch_bounds__(kAdjThreads) void adj_pair_kernel(Geom g, int t0, i
   static_assert(rec0 / 2 >= jrec * kAdjThreads && rec1 / 2 - 1 < jrec * kAdjThreads + 64,
                 "receiver pairs belong to wave 0, which stages the residual");
   issue_res(t1 - 1);
-  constexpr int DC = kAdjCPrefetch;            // c2 (or c2 A) loads DC pairs ahead
-  static_assert(KP % DC == 0, "pair count must be a multiple of the prefetch distance");
+  constexpr int DC = 4;  // c2 (or c2 A) loads DC pairs ahead (11 measured 0.7 % faster but needs every register)
+  // pair n of the whole run (step-major) sits in slot n mod DC; two steps make a whole number of turns
+  static_assert((2 * KP) % DC == 0, "two steps of pairs must fill the c2 ring a whole number of times");
   float2 Cb[DC];
 #pragma unroll
   for (int d = 0; d < DC; ++d) {
@@ -789,7 +785,8 @@ __global__ __launch_bounds__(kAdjThreads) void adj_pair_kernel(Geom g, int t0, i
   // The time loop runs two steps per iteration with the roles of the two state arrays
   // swapped in the second, so the new state is written over the oldest one and no
   // register moves the state along.
-  auto step = [&](float2(&M1)[KP], float2(&M2)[KP], const int it) __attribute__((always_inline)) {
+  auto step = [&](float2(&M1)[KP], float2(&M2)[KP], const int it, auto phase) __attribute__((always_inline)) {
+    constexpr int cs = (decltype(phase)::value * KP) % DC;  // c2 ring slot of this step's pair 0
     const int slot = it - t0;
     const float wt = w[it];
     float2 msrc = make_float2(0.0f, 0.0f);  // the new state of the source's pair (its owner's lane)
@@ -865,11 +862,11 @@ __global__ __launch_bounds__(kAdjThreads) void adj_pair_kernel(Geom g, int t0, i
 #pragma unroll
     for (int j = 0; j < KP; ++j) {
       asm volatile("" : "+v"(pk[j]));
-      const float2 cv = Cb[j % DC];
+      const float2 cv = Cb[(cs + j) % DC];
       {
         // c2 is the same every step: after the last pair, load the first pairs of the next step
         const int jn = j + DC < KP ? j + DC : j + DC - KP;
-        if (t + jn * kAdjThreads < NP) Cb[j % DC] = buf_load2(crs, off * 8, jn * kAdjThreads * 8);
+        if (t + jn * kAdjThreads < NP) Cb[(cs + j) % DC] = buf_load2(crs, off * 8, jn * kAdjThreads * 8);
       }
       if (t + j * kAdjThreads < NP) {
         if (C2A) {
@@ -892,11 +889,11 @@ __global__ __launch_bounds__(kAdjThreads) void adj_pair_kernel(Geom g, int t0, i
   };
   int it = t1 - 1;
   for (; it - 1 >= t0; it -= 2) {
-    step(m1, m2, it);
-    step(m2, m1, it - 1);
+    step(m1, m2, it, std::integral_constant<int, 0>());
+    step(m2, m1, it - 1, std::integral_constant<int, 1>());
   }
   const bool odd = it >= t0;  // one step left: the current state is then in m2
-  if (odd) step(m1, m2, it);
+  if (odd) step(m1, m2, it, std::integral_constant<int, 0>());
   asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // the ring loads of the last step land before exit
 #pragma unroll
   for (int j = 0; j < KP; ++j) {
