@@ -24,14 +24,15 @@
 // for model cells, the illumination index, plus interior and model flags),
 // two per register, decoded with two bit operations per step.
 //
-// The 150 x 150 plane (70 x 70 model, pad 40) runs paired kernels compiled
-// with its shape as constants (fwd_pair_kernel, adj_pair_kernel): a thread
-// owns cell pairs (2q, 2q+1), reads both cells' taps with 8-byte LDS accesses
-// and moves history, c2 and residual two cells per buffer instruction; their
-// descriptors hold the sponge-table byte offset directly, and the table sits
-// at LDS address 0 ((A, B) entries for the forward, (2A, A*B) for the adjoint,
-// A = 0.5 * 2A exactly).  Other shapes run the cell kernels with runtime
-// dimensions.
+// The 150 x 150 plane (70 x 70 model, pad 40) runs kernels compiled with its
+// shape as constants.  The forward (fwd_strip_kernel) gives each thread a
+// column pair in one 15-row strip with the strip's wavefield in registers.
+// The adjoint (adj_pair_kernel) has a thread own cell pairs (2q, 2q+1), read
+// both cells' taps with 8-byte LDS accesses and move history, c2 and residual
+// two cells per buffer instruction.  Their pair descriptors hold the
+// sponge-table byte offset directly, and the table sits at LDS address 0
+// ((A, B) entries for the forward, (2A, A*B) for the adjoint, A = 0.5 * 2A
+// exactly).  Other shapes run the cell kernels with runtime dimensions.
 #include <algorithm>
 
 #include "ss_device.h"
@@ -338,145 +339,185 @@ __global__ __launch_bounds__(kFwdThreads) void fwd_resident_kernel(Geom g, int t
 }
 
 
-// Paired forward for planes of even width: thread t owns the cell pairs
-// (2q, 2q+1), q = t + 1024 j, so its nine-tap stencils come from seven aligned
-// 8-byte LDS reads per pair (the row taps c-2..c+3 and the pairs two and one
-// rows above and below), one 16-byte read gives both cells' sponge
-// coefficients, and each Laplacian store moves two cells.  Per cell the
-// arithmetic is that of fwd_resident_kernel; every flag the forward needs
-// (interior, model, receiver row) starts on an even column, so a pair never
-// mixes the cases.
-template <int KP, int NZ, int NX, int PAD, int ROW>
-__global__ __launch_bounds__(kFwdThreads) void fwd_pair_kernel(Geom g, int t0, int t1, const float *__restrict__ c2,
-                                                               const float *__restrict__ A,
-                                                               const float *__restrict__ B,
-                                                               const float *__restrict__ w,
-                                                               const float *__restrict__ amp, float *__restrict__ p0g,
-                                                               float *__restrict__ p1g, float *__restrict__ gathers,
-                                                               float *__restrict__ hist, float *__restrict__ ckpt,
-                                                               int ckpt_every) {
+// Column-strip forward for the fixed 150 x 150 plane.  Thread t < 750 owns
+// the column pair (2c, 2c+1), c = t % 75, in the 15 rows r0 .. r0+14 of strip
+// t / 75 (r0 = 15 (t / 75)), and keeps p(t-1), p(t) and c2 of those rows in
+// registers: two register sets whose roles swap every step (the time loop runs
+// two steps per iteration), p(t+1) overwriting p(t-1).  A row's nine taps come
+// from registers except the two rows above and below the strip (four 8-byte
+// LDS reads per strip and step) and the left and right neighbour pairs (two
+// 8-byte reads per row, issued one row ahead together with the row's 16-byte
+// sponge entry), where reading all nine taps takes seven reads per pair.  After
+// every wave has read p(t) the strip publishes p(t+1) to LDS; the source owner
+// first adds the source to its register copy (gpu.py:114, after damping) and
+// the receiver row (gpu.py:134) is stored from registers.  The barriers order
+// LDS accesses only: the history, receiver and checkpoint stores are read by
+// later launches.  Lanes t >= 750 mirror thread 749's reads and store nothing;
+// buffer resources stay uniform (a per-lane resource turns every access into a
+// loop over lanes), idle lanes storing past every resource's size, which the
+// hardware drops.
+constexpr int kStripRows = 15, kStripThreads = 768;
+
+// One time step of a strip: PC holds p(t) of the strip's rows, PO p(t-1) on entry and
+// p(t+1) on exit.  Only the two rows above and below the strip and the left and right
+// neighbour pairs of every row come from LDS.
+template <int R, int W, int RK>
+__device__ __forceinline__ void strip_step(float2 (&PO)[R], float2 (&PC)[R], const float2 (&cc)[R],
+                                           const uint32_t (&tb)[R], const bool (&inner)[R], uint32_t a,
+                                           __amdgpu_buffer_rsrc_t hrs, int hv, int hoff,
+                                           bool src_owner, bool src_hi, float sv, bool rec_thread,
+                                           __amdgpu_buffer_rsrc_t grs, int gv, int goff, bool act) {
+  constexpr uint32_t kRow = 4u * W;
+  asm volatile("" : "+v"(a));
+  const float2 up2 = lds_ld2(a), up1 = lds_ld2(a + kRow);
+  const float2 dn1 = lds_ld2(a + (R + 2) * kRow), dn2 = lds_ld2(a + (R + 3) * kRow);
+  float2 lf = lds_ld2(a + 2 * kRow - 8), rt = lds_ld2(a + 2 * kRow + 8);
+  float4 ab = lds_ld4(tb[0]);
+#pragma unroll
+  for (int k = 0; k < R; ++k) {
+    asm volatile("" : "+v"(a));  // one row at a time
+    const uint32_t ar = a + (uint32_t)(k + 2) * kRow;
+    float2 lfn = make_float2(0.0f, 0.0f), rtn = lfn;
+    float4 abn = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (k + 1 < R) {  // next row's neighbours and coefficients, one row ahead
+      lfn = lds_ld2(ar + kRow - 8);
+      rtn = lds_ld2(ar + kRow + 8);
+      abn = lds_ld4(tb[k + 1]);
+    }
+    const float2 zm2 = k >= 2 ? PC[k >= 2 ? k - 2 : 0] : (k == 1 ? up1 : up2);
+    const float2 zm1 = k >= 1 ? PC[k >= 1 ? k - 1 : 0] : up1;
+    const float2 zc = PC[k];
+    const float2 zp1 = k + 1 < R ? PC[k + 1 < R ? k + 1 : 0] : dn1;
+    const float2 zp2 = k + 2 < R ? PC[k + 2 < R ? k + 2 : 0] : (k + 2 == R ? dn1 : dn2);
+    const float s0 = lap_sum(zm2.x, zm1.x, zc.x, zp1.x, zp2.x, lf.x, lf.y, zc.y, rt.x);
+    const float s1 = lap_sum(zm2.y, zm1.y, zc.y, zp1.y, zp2.y, lf.y, zc.x, rt.x, rt.y);
+    const float2 Lq = div12x2(s0, s1);
+    const float L0 = inner[k] ? Lq.x : 0.0f, L1 = inner[k] ? Lq.y : 0.0f;
+    PO[k].x = leapfrog(zc.x, PO[k].x, ab.y, cc[k].x, L0, ab.x);
+    PO[k].y = leapfrog(zc.y, PO[k].y, ab.w, cc[k].y, L1, ab.z);
+    buf_store2(make_float2(L0, L1), hrs, hv, hoff + k * (int)kRow);
+    asm volatile("" : "+v"(PO[k].x), "+v"(PO[k].y));  // finish this row before the next starts
+    lf = lfn;
+    rt = rtn;
+    ab = abn;
+  }
+  lds_barrier();
+  if (src_owner) {  // gpu.py:114, added after damping
+    if (src_hi)
+      PO[RK].y = PO[RK].y + sv;
+    else
+      PO[RK].x = PO[RK].x + sv;
+  }
+  if (rec_thread) buf_store2(PO[RK], grs, gv, goff);  // p(it+1) after injection
+  if (act) {
+#pragma unroll
+    for (int k = 0; k < R; ++k) lds_st2(a + (uint32_t)(k + 2) * kRow, PO[k]);
+  }
+  lds_barrier();
+}
+
+template <int NZ, int NX, int PAD, int ROW>
+__global__ __launch_bounds__(kStripThreads) void fwd_strip_kernel(Geom g, int t0, int t1, const float *__restrict__ c2,
+                                                                  const float *__restrict__ A,
+                                                                  const float *__restrict__ B,
+                                                                  const float *__restrict__ w,
+                                                                  const float *__restrict__ amp,
+                                                                  float *__restrict__ p0g, float *__restrict__ p1g,
+                                                                  float *__restrict__ gathers, float *__restrict__ hist,
+                                                                  float *__restrict__ ckpt, int ckpt_every) {
   extern __shared__ float lds[];
   constexpr int nz = NZ, nx = NX, pad = PAD, row = ROW;
-  constexpr int H = nz + 2 * pad, W = nx + 2 * pad;
-  constexpr int HW = H * W, NP = HW / 2;
+  constexpr int H = nz + 2 * pad, W = nx + 2 * pad, HW = H * W;
+  constexpr int R = kStripRows, NC = W / 2, NS = H / R, NACT = NC * NS;
   static_assert(W % 2 == 0 && nx % 2 == 0 && pad % 2 == 0, "pairs need even widths");
+  static_assert(H % R == 0 && NACT <= kStripThreads && R >= 2, "strips must tile the plane");
   static_assert((pad + 1) * (pad + 1) * 16 <= (int)kPTabMask + 16, "pair-table offsets must fit the descriptor");
   constexpr int guard = pair_guard(W), tabf = pair_table_floats(pad);
-  float4 *tab = reinterpret_cast<float4 *>(lds);  // at LDS address 0: lookups need no base add
+  constexpr int RS = (pad + row) / R, RK = (pad + row) % R;  // strip and slot of the receiver row
+  float4 *tab = reinterpret_cast<float4 *>(lds);
   float *P = lds + tabf + guard;
-  float2 *P2 = reinterpret_cast<float2 *>(P);
-  constexpr uint32_t kPAddr = (uint32_t)(tabf + guard) * 4;  // LDS byte address of P
+  constexpr uint32_t kPAddr = (uint32_t)(tabf + guard) * 4;
   const int64_t prop = xcd_prop(g);
-  if (prop < 0) return;  // padding workgroup (whole workgroup exits before any barrier)
+  if (prop < 0) return;
   const int b = (int)(prop / g.ns), s = (int)(prop - (int64_t)b * g.ns);
   const int t = threadIdx.x;
+  const bool act = t < NACT;
+  const int tt = act ? t : NACT - 1;
+  const int strip = tt / NC, c = tt - strip * NC, r0 = strip * R;
 
-  for (int k = t; k < guard; k += kFwdThreads) {
+  for (int k = t; k < guard; k += kStripThreads) {
     P[k - guard] = 0.0f;
     P[HW + k] = 0.0f;
   }
   fill_pair_table(tab, A, B, W, pad, 0);
 
-  float2 pn[KP], cc[KP];
-  uint32_t pk[KP];  // pair descriptors
-  // Laplacian-interior flag of a pair, kept as a lane mask: one select per cell.
-  bool inner[KP];
+  float2 Pa[R], Pb[R], cc[R];
+  uint32_t tb[R];
+  bool inner[R];
   const float2 *p0p = reinterpret_cast<const float2 *>(p0g + prop * HW);
   const float2 *p1p = reinterpret_cast<const float2 *>(p1g + prop * HW);
   const float2 *c2b = reinterpret_cast<const float2 *>(c2 + (int64_t)b * HW);
+  float2 *P2 = reinterpret_cast<float2 *>(P);
 #pragma unroll
-  for (int j = 0; j < KP; ++j) {
-    const int q = t + j * kFwdThreads;
-    pn[j] = cc[j] = make_float2(0.0f, 0.0f);
-    pk[j] = 0;
-    inner[j] = false;
-    if (q < NP) {
-      pn[j] = p0p[q];
-      P2[q] = p1p[q];
-      cc[j] = c2b[q];
-      pk[j] = pair_desc(q, H, W, nz, nx, pad);
-      inner[j] = (pk[j] & kPInterior) != 0;
-    }
+  for (int k = 0; k < R; ++k) {
+    const int q = (r0 + k) * NC + c;
+    Pa[k] = p0p[q];
+    Pb[k] = p1p[q];
+    cc[k] = c2b[q];
+    if (act) P2[q] = Pb[k];
+    const uint32_t d = pair_desc(q, H, W, nz, nx, pad);
+    tb[k] = d & kPTabMask;
+    inner[k] = (d & kPInterior) != 0;
   }
-  constexpr int rec0 = (pad + row) * W + pad;
-  const int srci = rec0 + g.src[s];
-  const bool src_owner = ((srci >> 1) % kFwdThreads) == t;
+  const int srcx = pad + g.src[s];
+  const bool src_owner = act && strip == RS && c == (srcx >> 1);
+  const bool src_hi = (srcx & 1) != 0;
   const float ampv = amp[prop];
+  const bool rec_thread = act && gathers && strip == RS && c >= pad / 2 && c < (pad + nx) / 2;
   const int nslots = t1 - t0;
   const int nck = ckpt ? (nslots + ckpt_every - 1) / ckpt_every : 0;
-  // receivers: thread t records columns t, t + 1024, ... of the receiver row
-  const bool rec_thread = gathers && t < nx;
-  // Without a history buffer the resource has size 0 and the hardware drops
-  // the Laplacian stores, so the step loop needs no branch for them.
   const __amdgpu_buffer_rsrc_t hrs =
       make_rsrc(hist ? hist + prop * nslots * HW : p0g, hist ? (int64_t)nslots * HW : 0);
-  const __amdgpu_buffer_rsrc_t grs =
-      make_rsrc(gathers ? gathers + prop * g.nt * nx : p0g, (int64_t)g.nt * nx);
+  const __amdgpu_buffer_rsrc_t grs = make_rsrc(gathers ? gathers + prop * g.nt * nx : p0g, (int64_t)g.nt * nx);
   const __amdgpu_buffer_rsrc_t crs = make_rsrc(ckpt ? ckpt + prop * nck * 2 * HW : p0g, nck * 2 * HW);
+  const int hv = act ? (r0 * W + 2 * c) * 4 : 0x40000000;
+  const int gv = (2 * c - pad) * 4;
+  const uint32_t a = kPAddr + (uint32_t)(((r0 - 2) * W + 2 * c) * 4);
   __syncthreads();
 
-  for (int it = t0; it < t1; ++it) {
-    const int slot = it - t0;
-    opaque(pk);
-    if (rec_thread && it > t0)  // p(it) after injection is the sample of step it-1
-      for (int x = t; x < nx; x += kFwdThreads) buf_store(P[rec0 + x], grs, x * 4, (it - 1) * nx * 4);
+  auto ckpt_store = [&](int slot, const float2(&older)[R], const float2(&newer)[R]) {
     if (ckpt && slot % ckpt_every == 0) {
-      const int base = (slot / ckpt_every) * 2 * HW;
+      const int base = (slot / ckpt_every) * 2 * HW * 4;
 #pragma unroll
-      for (int j = 0; j < KP; ++j) {
-        const int q = t + j * kFwdThreads;
-        if (q < NP) {
-          buf_store2(pn[j], crs, t * 8, (base + 2 * j * kFwdThreads) * 4);
-          buf_store2(P2[q], crs, t * 8, (base + HW + 2 * j * kFwdThreads) * 4);
-        }
+      for (int k = 0; k < R; ++k) {
+        buf_store2(older[k], crs, hv, base + k * W * 4);
+        buf_store2(newer[k], crs, hv, base + (HW + k * W) * 4);
       }
     }
-    int off = t;
-#pragma unroll
-    for (int j = 0; j < KP; ++j) {
-      asm volatile("" : "+v"(pk[j]));  // one pair at a time
-      if (t + j * kFwdThreads < NP) {
-        // One base address per pair (two rows up); the seven taps are
-        // immediate offsets from it.
-        uint32_t a = kPAddr + 8u * (uint32_t)(off + j * kFwdThreads - W);
-        asm volatile("" : "+v"(a));
-        const float2 u2m = lds_ld2(a), um = lds_ld2(a + 4 * W), r0 = lds_ld2(a + 8 * W - 8);
-        const float2 r1 = lds_ld2(a + 8 * W), r2 = lds_ld2(a + 8 * W + 8), up = lds_ld2(a + 12 * W);
-        const float2 u2p = lds_ld2(a + 16 * W);
-        const float s0 = lap_sum(u2m.x, um.x, r1.x, up.x, u2p.x, r0.x, r0.y, r1.y, r2.x);
-        const float s1 = lap_sum(u2m.y, um.y, r1.y, up.y, u2p.y, r0.y, r1.x, r2.x, r2.y);
-        const float2 Lq = div12x2(s0, s1);
-        const float L0 = inner[j] ? Lq.x : 0.0f, L1 = inner[j] ? Lq.y : 0.0f;
-        const float4 ab = pair_coef(pk[j]);  // (A0, B0, A1, B1)
-        pn[j].x = leapfrog(r1.x, pn[j].x, ab.y, cc[j].x, L0, ab.x);
-        pn[j].y = leapfrog(r1.y, pn[j].y, ab.w, cc[j].y, L1, ab.z);
-        buf_store2(make_float2(L0, L1), hrs, off * 8, (slot * HW + 2 * j * kFwdThreads) * 4);
-        asm volatile("" : "+v"(pn[j].x), "+v"(pn[j].y));  // finish this pair before the next starts
-      }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int j = 0; j < KP; ++j) {
-      const int q = t + j * kFwdThreads;
-      if (q < NP) {
-        const float2 c = P2[q];
-        P2[q] = pn[j];
-        pn[j] = c;
-      }
-    }
-    if (src_owner) P[srci] = P[srci] + w[it] * ampv;  // gpu.py:114, added after damping
-    __syncthreads();
+  };
+  int it = t0;
+  for (; it + 1 < t1; it += 2) {
+    ckpt_store(it - t0, Pa, Pb);
+    strip_step<R, W, RK>(Pa, Pb, cc, tb, inner, a, hrs, hv, (it - t0) * HW * 4, src_owner, src_hi,
+                         w[it] * ampv, rec_thread, grs, gv, it * nx * 4, act);
+    ckpt_store(it + 1 - t0, Pb, Pa);
+    strip_step<R, W, RK>(Pb, Pa, cc, tb, inner, a, hrs, hv, (it + 1 - t0) * HW * 4, src_owner, src_hi,
+                         w[it + 1] * ampv, rec_thread, grs, gv, (it + 1) * nx * 4, act);
   }
-  if (rec_thread)
-    for (int x = t; x < nx; x += kFwdThreads) buf_store(P[rec0 + x], grs, x * 4, (t1 - 1) * nx * 4);
-  float2 *p0o = reinterpret_cast<float2 *>(p0g + prop * HW);
-  float2 *p1o = reinterpret_cast<float2 *>(p1g + prop * HW);
+  const bool odd = it < t1;
+  if (odd) {
+    ckpt_store(it - t0, Pa, Pb);
+    strip_step<R, W, RK>(Pa, Pb, cc, tb, inner, a, hrs, hv, (it - t0) * HW * 4, src_owner, src_hi,
+                         w[it] * ampv, rec_thread, grs, gv, it * nx * 4, act);
+  }
+  if (act) {
+    float2 *p0o = reinterpret_cast<float2 *>(p0g + prop * HW);
+    float2 *p1o = reinterpret_cast<float2 *>(p1g + prop * HW);
 #pragma unroll
-  for (int j = 0; j < KP; ++j) {
-    const int q = t + j * kFwdThreads;
-    if (q < NP) {
-      p0o[q] = pn[j];
-      p1o[q] = P2[q];
+    for (int k = 0; k < R; ++k) {
+      const int q = (r0 + k) * NC + c;
+      p0o[q] = odd ? Pb[k] : Pa[k];
+      p1o[q] = odd ? Pa[k] : Pb[k];
     }
   }
 }
@@ -987,12 +1028,13 @@ int forward_resident(const Geom &g, int t0, int t1, const float *c2, const float
   return dispatch_fwd(g, [&](auto kc, auto fixed) -> int {
     constexpr bool F = decltype(fixed)::value;
     // the fixed 150 x 150 plane runs the paired kernel (11 pairs per thread)
-    auto kern = F ? fwd_pair_kernel<(150 * 150 / 2 + kFwdThreads - 1) / kFwdThreads, 70, 70, 40, 1>
+    auto kern = F ? fwd_strip_kernel<70, 70, 40, 1>
                   : fwd_resident_kernel<decltype(kc)::value, F ? 70 : 0, F ? 70 : 0, F ? 40 : 0, F ? 1 : -1>;
     hipError_t e = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return hip_status(e, "forward_resident attribute");
     if (F && !no_static_lds((const void *)kern)) return hip_status(hipErrorInvalidValue, "forward_resident static LDS");
-    kern<<<dim3((unsigned)xcd_blocks((int64_t)g.nb * g.ns)), dim3(kFwdThreads), lds, st>>>(g, t0, t1, c2, A, B, w, amp, p0,
+    kern<<<dim3((unsigned)xcd_blocks((int64_t)g.nb * g.ns)), dim3(F ? kStripThreads : kFwdThreads), lds, st>>>(
+        g, t0, t1, c2, A, B, w, amp, p0,
                                                                                  p1, gathers, hist, ckpt, ckpt_every);
     return hip_status(hipGetLastError(), "forward_resident launch");
   });
