@@ -80,11 +80,14 @@ __host__ __device__ __forceinline__ T lapt_sum(T qc, T qzp2, T qzp1, T qzm1, T q
 }
 
 // Correctly rounded s/12 from the reciprocal product and one FMA correction.
-// Whenever that result is zero or normal it equals IEEE division; it can be
-// wrong only when it is subnormal (|s| below about 2^-122), and it is NaN for
-// infinite s.  Those cases take IEEE division, chosen by one class test on the
-// result (checked on the host for all 2^32 inputs by tests/native/div12_check.cpp).
-// Exact zeros, common ahead of the wavefront, stay on the fast path.
+// Whenever that result is zero or normal it equals IEEE division.  When it is
+// subnormal (|s| below about 2^-122) it can be wrong only at an exact tie of
+// the subnormal grid, s/12 = (n + 1/2) 2^-149, which the FMA rounds to the
+// neighbour the correction's sign points at rather than to the even one; for
+// infinite s it is NaN.  One class test on the result sends those cases to
+// div12_fix (checked on the host for all 2^32 inputs by
+// tests/native/div12_check.cpp).  Exact zeros, common ahead of the wavefront,
+// stay on the fast path.
 constexpr int kZeroOrNormal = 0x168;  // __builtin_isfpclass: -normal, -zero, +zero, +normal
 
 // The remainder is formed as -(12 q0 - s) rather than s - 12 q0: the same value
@@ -97,21 +100,35 @@ __host__ __device__ __forceinline__ float div12_fast(float s) {
   return __builtin_fmaf(r, R, q0);
 }
 
+// q = div12_fast(s) when it is neither zero nor normal.  Subnormal q: d = 12 q - s
+// is exact (both are multiples of 2^-149 and d is small); d = +-6 * 2^-149 marks
+// an exact tie, and an odd q then moves one grid step towards s/12, to its even
+// neighbour: q - d * RN(1/6) is q -+ 2^-149 plus a perturbation far below half a
+// grid step, so the FMA rounds it to q -+ 2^-149 exactly.  Non-finite q (infinite
+// or NaN s): s * RN(1/12) is +-inf or NaN, as IEEE division gives.
+__host__ __device__ __forceinline__ float div12_fix(float s, float q) {
+  const float d = __builtin_fmaf(q, 12.0f, -s);
+  const bool tie = __builtin_fabsf(d) == 0x1.8p-147f && (__builtin_bit_cast(uint32_t, q) & 1u) != 0u;
+  q = tie ? __builtin_fmaf(-d, 0x1.555556p-3f, q) : q;
+  if (q != q) q = s * 0x1.555556p-4f;
+  return q;
+}
+
 __host__ __device__ __forceinline__ float div12(float s) {
   float q = div12_fast(s);
-  if (!__builtin_isfpclass(q, kZeroOrNormal)) q = s / 12.0f;
+  if (!__builtin_isfpclass(q, kZeroOrNormal)) q = div12_fix(s, q);
   return q;
 }
 
 // div12 of two values with one branch: both keep the fast result unless either
-// needs IEEE division, and then each falls back only if it needs to.  Results
-// equal div12(s0), div12(s1).
+// needs the fix, and then each is fixed only if it needs it.  Results equal
+// div12(s0), div12(s1).
 __host__ __device__ __forceinline__ float2 div12x2(float s0, float s1) {
   float2 q = make_float2(div12_fast(s0), div12_fast(s1));
   const bool ok0 = __builtin_isfpclass(q.x, kZeroOrNormal), ok1 = __builtin_isfpclass(q.y, kZeroOrNormal);
   if (!(ok0 && ok1)) {
-    if (!ok0) q.x = s0 / 12.0f;
-    if (!ok1) q.y = s1 / 12.0f;
+    if (!ok0) q.x = div12_fix(s0, q.x);
+    if (!ok1) q.y = div12_fix(s1, q.y);
   }
   return q;
 }

@@ -34,7 +34,8 @@ static uint64_t check_range(uint64_t lo, uint64_t hi) {
 static uint64_t check_pairs() {
   const float vals[] = {0.0f,        -0.0f,      1.0f,       -3.5f,     0x1p-119f, 0x1.fffffep-120f, 0x1p-140f,
                         -0x1p-149f,  0x1p-126f,  0x1p-122f,  0x1p-123f, 1e30f,     0x1.fffffep+127f, INFINITY,
-                        -INFINITY,   NAN,        12.0f,      7.25e-20f};
+                        -INFINITY,   NAN,        12.0f,      7.25e-20f,
+                        0x1.8p-147f, -0x1.2p-145f, 0x1.ep-145f, -0x1.5p-144f};  // exact subnormal ties
   uint64_t bad = 0;
   for (float a : vals)
     for (float b : vals) {
