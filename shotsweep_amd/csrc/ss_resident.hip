@@ -948,6 +948,211 @@ __global__ __launch_bounds__(kAdjThreads) void adj_pair_kernel(Geom g, int t0, i
   for (int k = t; k < nzx; k += kAdjThreads) illumg[prop * nzx + k] = il[k];
 }
 
+// Column-strip adjoint for the fixed 150 x 150 plane with the c2 * A plane
+// (c2a_kernel: zero on the two-cell rim, so q is zero there, as lap_adjoint
+// reads only p[2:-2, 2:-2], gpu.py:41).  The strip layout of fwd_strip_kernel:
+// thread t < 750 owns column pair c = t % 75 in rows r0 .. r0+14 of strip t / 75
+// and keeps m1, m2 (two register sets swapping roles every step, as the state
+// arrays of the reference's recursion shift) and the imaging sum of those rows in
+// registers.  LDS holds q = (c2 A m1) / 12, the operand of lap_adjoint, the
+// model-cell illumination and the sponge table of (2A0, A0 B0, 2A1, A1 B1)
+// entries.  Phase 1 walks the strip top to bottom with the centre q pairs of rows
+// r-2 .. r+2 in registers (one new 8-byte read per row) plus the row's left and
+// right neighbour pairs, and reads the row's sponge entry and forward Laplacian
+// ahead of use (history three rows ahead into registers): mu = 2A m1 + lapT(q) -
+// A B m2 (gpu.py:179), the residual on the receiver row (gpu.py:180), the imaging
+// sum (A mu) L and, on model cells, the illumination L^2 (gpu.py:181, 183), the
+// source term once per step (gpu.py:182).  Phase 2, after every wave has read q,
+// publishes the next step's q from the new state, c2 A loaded four rows ahead from
+// the L2-resident plane.  Lanes t >= 750 mirror thread 749's reads and store
+// nothing.
+template <int NZ, int NX, int PAD, int ROW>
+__global__ __launch_bounds__(kStripThreads) void adj_strip_kernel(Geom g, int t0, int t1,
+                                                                  const float *__restrict__ c2a,
+                                                                  const float *__restrict__ A,
+                                                                  const float *__restrict__ B,
+                                                                  const float *__restrict__ w,
+                                                                  const float *__restrict__ sd,
+                                                                  const float *__restrict__ res,
+                                                                  const float *__restrict__ hist,
+                                                                  float *__restrict__ m1g, float *__restrict__ m2g,
+                                                                  float *__restrict__ gaccg,
+                                                                  float *__restrict__ illumg,
+                                                                  float *__restrict__ src_grad) {
+  extern __shared__ float lds[];
+  constexpr int nz = NZ, nx = NX, pad = PAD, row = ROW;
+  constexpr int H = nz + 2 * pad, W = nx + 2 * pad, HW = H * W;
+  constexpr int R = kStripRows, NC = W / 2, NS = H / R, NACT = NC * NS;
+  static_assert(W % 2 == 0 && nx % 2 == 0 && pad % 2 == 0, "pairs need even widths");
+  static_assert(H % R == 0 && NACT <= kStripThreads && R >= 4, "strips must tile the plane");
+  static_assert((pad + 1) * (pad + 1) * 16 <= (int)kPTabMask + 16, "pair-table offsets must fit the descriptor");
+  constexpr int nzx = nz * nx;
+  constexpr int guard = pair_guard(W), tabf = pair_table_floats(pad);
+  constexpr int RS = (pad + row) / R, RK = (pad + row) % R;  // strip and slot of the receiver row
+  constexpr uint32_t kRow = 4u * W;
+  constexpr int DH = 3, DC = 4;  // history and c2 A rows loaded ahead
+  float4 *tab = reinterpret_cast<float4 *>(lds);
+  float *il = lds + tabf;
+  constexpr uint32_t kIlAddr = (uint32_t)tabf * 4;
+  float *Q = il + nzx + guard;
+  float2 *Q2 = reinterpret_cast<float2 *>(Q);
+  constexpr uint32_t kQAddr = (uint32_t)(tabf + nzx + guard) * 4;
+  const int64_t prop = xcd_prop(g);
+  if (prop < 0) return;
+  const int b = (int)(prop / g.ns), s = (int)(prop - (int64_t)b * g.ns);
+  const int t = threadIdx.x;
+  const bool act = t < NACT;
+  const int tt = act ? t : NACT - 1;
+  const int strip = tt / NC, c = tt - strip * NC, r0 = strip * R;
+
+  for (int k = t; k < guard; k += kStripThreads) {
+    Q[k - guard] = 0.0f;
+    Q[HW + k] = 0.0f;
+  }
+  fill_pair_table(tab, A, B, W, pad, 1);
+  for (int k = t; k < nzx; k += kStripThreads) il[k] = illumg[prop * nzx + k];
+
+  float2 m1[R], m2[R], ga[R];
+  uint32_t tb[R];
+  bool mdl[R];
+  const bool colmodel = act && c >= pad / 2 && c < (pad + nx) / 2;
+  const float2 *m1p = reinterpret_cast<const float2 *>(m1g + prop * HW);
+  const float2 *m2p = reinterpret_cast<const float2 *>(m2g + prop * HW);
+  const float2 *gap = reinterpret_cast<const float2 *>(gaccg + prop * HW);
+  const float2 *cap = reinterpret_cast<const float2 *>(c2a + (int64_t)b * HW);
+#pragma unroll
+  for (int k = 0; k < R; ++k) {
+    const int q = (r0 + k) * NC + c;
+    m1[k] = m1p[q];
+    m2[k] = m2p[q];
+    ga[k] = gap[q];
+    if (act) {
+      const float2 cv = cap[q];
+      Q2[q] = div12x2(cv.x * m1[k].x, cv.y * m1[k].y);
+    }
+    tb[k] = pair_desc(q, H, W, nz, nx, pad) & kPTabMask;
+    mdl[k] = colmodel && r0 + k >= pad && r0 + k < pad + nz;
+  }
+  const int srcx = pad + g.src[s];
+  const bool own_src = act && strip == RS && c == (srcx >> 1);
+  const bool src_hi = (srcx & 1) != 0;
+  const float sdv = sd[prop];
+  float sg = own_src ? src_grad[prop] : 0.0f;
+  const bool rec_thread = act && strip == RS && c >= pad / 2 && c < (pad + nx) / 2;
+  const int nslots = t1 - t0;
+  // uniform resources; lanes with nothing to load read past the resource's end (zeros)
+  const __amdgpu_buffer_rsrc_t rrs = make_rsrc(res + prop * g.nt * nx, (int64_t)g.nt * nx);
+  const __amdgpu_buffer_rsrc_t hrs = make_rsrc(hist + prop * nslots * HW, (int64_t)nslots * HW);
+  const __amdgpu_buffer_rsrc_t crs = make_rsrc(c2a + (int64_t)b * HW, (int64_t)HW);
+  const int hv = (r0 * W + 2 * c) * 4;              // the thread's first pair in a plane (bytes)
+  const int rv_off = rec_thread ? (2 * c - pad) * 4 : 0x40000000;
+  const uint32_t a = kQAddr + (uint32_t)(((r0 - 2) * W + 2 * c) * 4);
+  const uint32_t ia0 = kIlAddr + (uint32_t)(((r0 - pad) * nx + 2 * c - pad) * 4);  // model rows only
+  __syncthreads();
+
+  auto step = [&](float2(&M1)[R], float2(&M2)[R], const int it) __attribute__((always_inline)) {
+    const int slot = it - t0;
+    const float wt = w[it];
+    const float2 rv = buf_load2(rrs, rv_off, it * nx * 4);  // receiver lanes: the residual pair
+    float2 Lh[DH];
+#pragma unroll
+    for (int d = 0; d < DH; ++d) Lh[d] = buf_load2(hrs, hv, slot * HW * 4 + d * (int)kRow);
+    uint32_t aa = a;
+    asm volatile("" : "+v"(aa));
+    float2 z0 = lds_ld2(aa), z1 = lds_ld2(aa + kRow), z2 = lds_ld2(aa + 2 * kRow), z3 = lds_ld2(aa + 3 * kRow);
+    float2 z4 = lds_ld2(aa + 4 * kRow);
+    float2 lf = lds_ld2(aa + 2 * kRow - 8), rt = lds_ld2(aa + 2 * kRow + 8);
+    float4 e = lds_ld4(tb[0]);
+    float2 msrc = make_float2(0.0f, 0.0f);
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+      asm volatile("" : "+v"(aa));  // one row at a time
+      const uint32_t ar = aa + (uint32_t)(k + 2) * kRow;
+      float2 z5 = make_float2(0.0f, 0.0f), lfn = z5, rtn = z5;
+      float4 en = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      if (k + 1 < R) {  // next row's centre (row r+3), neighbours and coefficients, one row ahead
+        z5 = lds_ld2(ar + 3 * kRow);
+        lfn = lds_ld2(ar + kRow - 8);
+        rtn = lds_ld2(ar + kRow + 8);
+        en = lds_ld4(tb[k + 1]);
+      }
+      const float2 L = Lh[k % DH];
+      if (k + DH < R) Lh[k % DH] = buf_load2(hrs, hv, slot * HW * 4 + (k + DH) * (int)kRow);
+      const float lt0 = lapt_sum(z2.x, z4.x, z3.x, z1.x, z0.x, rt.x, z2.y, lf.y, lf.x);
+      const float lt1 = lapt_sum(z2.y, z4.y, z3.y, z1.y, z0.y, rt.y, rt.x, z2.x, lf.y);
+      float mu0 = e.x * M1[k].x;
+      float mu1 = e.z * M1[k].y;
+      mu0 = mu0 + lt0;
+      mu1 = mu1 + lt1;
+      mu0 = mu0 - e.y * M2[k].x;
+      mu1 = mu1 - e.w * M2[k].y;
+      if (k == RK && rec_thread) {  // gpu.py:180
+        mu0 = mu0 + rv.x;
+        mu1 = mu1 + rv.y;
+      }
+      ga[k].x = ga[k].x + ((0.5f * e.x) * mu0) * L.x;  // A = 0.5 (2A), exact
+      ga[k].y = ga[k].y + ((0.5f * e.z) * mu1) * L.y;
+      if (k == RK) msrc = make_float2(mu0, mu1);
+      if (mdl[k]) {
+        const uint32_t ia = ia0 + (uint32_t)(k * nx * 4);
+        float2 v = lds_ld2(ia);
+        v.x = v.x + L.x * L.x;
+        v.y = v.y + L.y * L.y;
+        lds_st2(ia, v);
+      }
+      M2[k] = make_float2(mu0, mu1);  // the new state; M1 (the old state) becomes the next step's m2
+      asm volatile("" : "+v"(ga[k].x), "+v"(ga[k].y), "+v"(M2[k].x), "+v"(M2[k].y));
+      z0 = z1;
+      z1 = z2;
+      z2 = z3;
+      z3 = z4;
+      z4 = z5;
+      lf = lfn;
+      rt = rtn;
+      e = en;
+    }
+    if (own_src) sg = sg + ((src_hi ? msrc.y : msrc.x) * wt) * sdv;  // gpu.py:182, in step order
+    float2 cb[DC];
+#pragma unroll
+    for (int d = 0; d < DC; ++d) cb[d] = buf_load2(crs, hv, d * (int)kRow);
+    lds_barrier();
+    if (act) {
+#pragma unroll
+      for (int k = 0; k < R; ++k) {
+        const float2 cv = cb[k % DC];
+        if (k + DC < R) cb[k % DC] = buf_load2(crs, hv, (k + DC) * (int)kRow);
+        lds_st2(aa + (uint32_t)(k + 2) * kRow, div12x2(cv.x * M2[k].x, cv.y * M2[k].y));
+      }
+    }
+    lds_barrier();
+  };
+  int it = t1 - 1;
+  for (; it - 1 >= t0; it -= 2) {
+    step(m1, m2, it);
+    step(m2, m1, it - 1);
+  }
+  const bool odd = it >= t0;  // one step left: the current state is then in m2
+  if (odd) step(m1, m2, it);
+  if (act) {
+    float2 *m1o = reinterpret_cast<float2 *>(m1g + prop * HW);
+    float2 *m2o = reinterpret_cast<float2 *>(m2g + prop * HW);
+    float2 *gao = reinterpret_cast<float2 *>(gaccg + prop * HW);
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+      const int q = (r0 + k) * NC + c;
+      m1o[q] = odd ? m2[k] : m1[k];
+      m2o[q] = odd ? m1[k] : m2[k];
+      gao[q] = ga[k];
+    }
+  }
+  if (own_src) src_grad[prop] = sg;
+  for (int k = t; k < nzx; k += kStripThreads) illumg[prop * nzx + k] = il[k];
+}
+
+size_t adj_strip_lds_bytes(const Geom &g) {
+  return (size_t)(pair_table_floats(g.pad) + g.nz * g.nx + g.H * g.W + 2 * pair_guard(g.W)) * sizeof(float);
+}
+
 // c2a = c2 * A on the interior cells of every plane of c2 (nb, H, W), zero on the
 // two-cell rim: the first product of the adjoint's q = ((c2 A) m1) / 12 (gpu.py:179),
 // rounded once as there.  Zero on the rim makes q zero there, as lap_adjoint only
@@ -1043,19 +1248,19 @@ int forward_resident(const Geom &g, int t0, int t1, const float *c2, const float
 int adjoint_resident(const Geom &g, int t0, int t1, const float *c2, const float *A, const float *B, const float *w,
                      const float *sd, const float *res, const float *hist, float *m1, float *m2, float *gacc,
                      float *illum, float *src_grad, hipStream_t st, const float *c2a) {
-  const size_t lds = is_openfwi(g) ? adj_pair_lds_bytes(g) : adj_lds_bytes(g);
+  const bool pre = is_openfwi(g) && c2a != nullptr;
+  const size_t lds = pre ? adj_strip_lds_bytes(g) : (is_openfwi(g) ? adj_pair_lds_bytes(g) : adj_lds_bytes(g));
   return dispatch_adj(g, [&](auto kc, auto fixed) -> int {
     constexpr bool F = decltype(fixed)::value;
-    // the fixed 150 x 150 plane runs the paired kernel (22 pairs per thread), with the
-    // c2 * A plane when the caller has one
+    // the fixed 150 x 150 plane runs the column-strip kernel when the caller has the
+    // c2 * A plane, else the paired kernel (22 pairs per thread)
     constexpr int KP = kAdjPairs150;
-    const bool pre = F && c2a != nullptr;
-    auto kern = F ? (pre ? adj_pair_kernel<KP, 70, 70, 40, 1, true> : adj_pair_kernel<KP, 70, 70, 40, 1, false>)
+    auto kern = F ? (pre ? adj_strip_kernel<70, 70, 40, 1> : adj_pair_kernel<KP, 70, 70, 40, 1, false>)
                   : adj_resident_kernel<decltype(kc)::value, F ? 70 : 0, F ? 70 : 0, F ? 40 : 0, F ? 1 : -1>;
     hipError_t e = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return hip_status(e, "adjoint_resident attribute");
     if (F && !no_static_lds((const void *)kern)) return hip_status(hipErrorInvalidValue, "adjoint_resident static LDS");
-    kern<<<dim3((unsigned)xcd_blocks((int64_t)g.nb * g.ns)), dim3(kAdjThreads), lds, st>>>(
+    kern<<<dim3((unsigned)xcd_blocks((int64_t)g.nb * g.ns)), dim3(pre ? kStripThreads : kAdjThreads), lds, st>>>(
         g, t0, t1, pre ? c2a : c2, A, B, w, sd, res, hist, m1, m2, gacc, illum, src_grad);
     return hip_status(hipGetLastError(), "adjoint_resident launch");
   });
