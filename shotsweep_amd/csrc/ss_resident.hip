@@ -963,9 +963,10 @@ __global__ __launch_bounds__(kAdjThreads) void adj_pair_kernel(Geom g, int t0, i
 // A B m2 (gpu.py:179), the residual on the receiver row (gpu.py:180), the imaging
 // sum (A mu) L and, on model cells, the illumination L^2 (gpu.py:181, 183), the
 // source term once per step (gpu.py:182).  Phase 2, after every wave has read q,
-// publishes the next step's q from the new state, c2 A loaded four rows ahead from
-// the L2-resident plane.  Lanes t >= 750 mirror thread 749's reads and store
-// nothing.
+// publishes the next step's q from the new state, c2 A loaded through an
+// eight-row ring from the L2-resident plane.  Lanes t >= 750 mirror thread 749
+// (same reads, same state); they write nothing but phase 2's q, whose values
+// equal thread 749's.
 template <int NZ, int NX, int PAD, int ROW>
 __global__ __launch_bounds__(kStripThreads) void adj_strip_kernel(Geom g, int t0, int t1,
                                                                   const float *__restrict__ c2a,
@@ -990,7 +991,7 @@ __global__ __launch_bounds__(kStripThreads) void adj_strip_kernel(Geom g, int t0
   constexpr int guard = pair_guard(W), tabf = pair_table_floats(pad);
   constexpr int RS = (pad + row) / R, RK = (pad + row) % R;  // strip and slot of the receiver row
   constexpr uint32_t kRow = 4u * W;
-  constexpr int DH = 3, DC = 4;  // history and c2 A rows loaded ahead
+  constexpr int DH = 3, DC = 8;  // history rows loaded ahead; depth of the c2 A ring
   float4 *tab = reinterpret_cast<float4 *>(lds);
   float *il = lds + tabf;
   constexpr uint32_t kIlAddr = (uint32_t)tabf * 4;
@@ -1045,18 +1046,24 @@ __global__ __launch_bounds__(kStripThreads) void adj_strip_kernel(Geom g, int t0
   const __amdgpu_buffer_rsrc_t hrs = make_rsrc(hist + prop * nslots * HW, (int64_t)nslots * HW);
   const __amdgpu_buffer_rsrc_t crs = make_rsrc(c2a + (int64_t)b * HW, (int64_t)HW);
   const int hv = (r0 * W + 2 * c) * 4;              // the thread's first pair in a plane (bytes)
-  const int rv_off = rec_thread ? (2 * c - pad) * 4 : 0x40000000;
   const uint32_t a = kQAddr + (uint32_t)(((r0 - 2) * W + 2 * c) * 4);
   const uint32_t ia0 = kIlAddr + (uint32_t)(((r0 - pad) * nx + 2 * c - pad) * 4);  // model rows only
   __syncthreads();
 
+  // history of a step's first DH rows and the receiver lanes' residual pair, loaded at
+  // the end of the previous step
+  float2 Lh[DH], rv;
+  auto load_step = [&](int it) __attribute__((always_inline)) {
+#pragma unroll
+    for (int d = 0; d < DH; ++d) Lh[d] = buf_load2(hrs, hv, (it - t0) * HW * 4 + d * (int)kRow);
+    // receiver lanes (strip RS) read their pair of the residual row; other lanes read
+    // some other in-range word or past the resource (zero), which they never use
+    rv = buf_load2(rrs, hv - (RS * R * W + pad) * 4, it * nx * 4);
+  };
+  load_step(t1 - 1);
   auto step = [&](float2(&M1)[R], float2(&M2)[R], const int it) __attribute__((always_inline)) {
     const int slot = it - t0;
     const float wt = w[it];
-    const float2 rv = buf_load2(rrs, rv_off, it * nx * 4);  // receiver lanes: the residual pair
-    float2 Lh[DH];
-#pragma unroll
-    for (int d = 0; d < DH; ++d) Lh[d] = buf_load2(hrs, hv, slot * HW * 4 + d * (int)kRow);
     uint32_t aa = a;
     asm volatile("" : "+v"(aa));
     float2 z0 = lds_ld2(aa), z1 = lds_ld2(aa + kRow), z2 = lds_ld2(aa + 2 * kRow), z3 = lds_ld2(aa + 3 * kRow);
@@ -1112,18 +1119,22 @@ __global__ __launch_bounds__(kStripThreads) void adj_strip_kernel(Geom g, int t0
       e = en;
     }
     if (own_src) sg = sg + ((src_hi ? msrc.y : msrc.x) * wt) * sdv;  // gpu.py:182, in step order
+    // Loads complete in issue order (one counter for all of them), so a wait for a
+    // c2 A row also waits for every history load issued before it: phase 2 reads c2 A
+    // from a ring DC rows deep whose first DC loads are issued before the barrier, and
+    // the next step's first history rows are issued only after phase 2.
     float2 cb[DC];
 #pragma unroll
-    for (int d = 0; d < DC; ++d) cb[d] = buf_load2(crs, hv, d * (int)kRow);
+    for (int k = 0; k < DC; ++k) cb[k] = buf_load2(crs, hv, k * (int)kRow);
     lds_barrier();
-    if (act) {
+    // Idle lanes store too: they mirror thread 749 and write its values to its cells.
 #pragma unroll
-      for (int k = 0; k < R; ++k) {
-        const float2 cv = cb[k % DC];
-        if (k + DC < R) cb[k % DC] = buf_load2(crs, hv, (k + DC) * (int)kRow);
-        lds_st2(aa + (uint32_t)(k + 2) * kRow, div12x2(cv.x * M2[k].x, cv.y * M2[k].y));
-      }
+    for (int k = 0; k < R; ++k) {
+      const float2 cv = cb[k % DC];
+      if (k + DC < R) cb[k % DC] = buf_load2(crs, hv, (k + DC) * (int)kRow);
+      lds_st2(aa + (uint32_t)(k + 2) * kRow, div12x2(cv.x * M2[k].x, cv.y * M2[k].y));
     }
+    if (it - 1 >= t0) load_step(it - 1);
     lds_barrier();
   };
   int it = t1 - 1;
