@@ -24,15 +24,13 @@
 // for model cells, the illumination index, plus interior and model flags),
 // two per register, decoded with two bit operations per step.
 //
-// The 150 x 150 plane (70 x 70 model, pad 40) runs kernels compiled with its
-// shape as constants.  The forward (fwd_strip_kernel) gives each thread a
-// column pair in one 15-row strip with the strip's wavefield in registers.
-// The adjoint (adj_pair_kernel) has a thread own cell pairs (2q, 2q+1), read
-// both cells' taps with 8-byte LDS accesses and move history, c2 and residual
-// two cells per buffer instruction.  Their pair descriptors hold the
-// sponge-table byte offset directly, and the table sits at LDS address 0
-// ((A, B) entries for the forward, (2A, A*B) for the adjoint, A = 0.5 * 2A
-// exactly).  Other shapes run the cell kernels with runtime dimensions.
+// The 150 x 150 plane (70 x 70 model, pad 40) runs column-strip kernels
+// compiled with its shape as constants (fwd_strip_kernel, adj_strip_kernel):
+// each thread owns one column pair (2c, 2c+1) in one 15-row strip and keeps
+// the strip's state in registers.  Their pair descriptors hold the sponge-table
+// byte offset directly, and the table sits at LDS address 0 ((A, B) entries for
+// the forward, (2A, A*B) for the adjoint, A = 0.5 * 2A exactly).  Other shapes
+// run the cell kernels with runtime dimensions.
 #include <algorithm>
 
 #include "ss_device.h"
@@ -56,9 +54,6 @@ constexpr int kAdjThreads = SS_ADJ_THREADS;  // adjoint: m1, m2, imaging sum = 3
 constexpr int kFwdMaxSlots = 24;
 constexpr int kAdjMaxSlots = 48;
 constexpr int kPrefetch = SS_PREFETCH;       // adjoint: Laplacian loads issued this many cells ahead
-// pairs per thread of the paired adjoint on the 150 x 150 plane (pairs beyond the plane are
-// skipped at compile time)
-constexpr int kAdjPairs150 = (150 * 150 / 2 + kAdjThreads - 1) / kAdjThreads;
 // slots of the 150 x 150 plane, rounded up to a multiple of the prefetch distance
 constexpr int kAdj150 = ((150 * 150 + kAdjThreads - 1) / kAdjThreads + kPrefetch - 1) / kPrefetch * kPrefetch;
 
@@ -115,7 +110,6 @@ __device__ __forceinline__ uint32_t pair_desc(int q, int H, int W, int nz, int n
     v |= 0x80000000u | ((uint32_t)(((z - pad) * nx + (x - pad)) * 4) << 16);
   return v;
 }
-__device__ __forceinline__ float4 pair_coef(uint32_t d) { return lds_ld4(d & kPTabMask); }
 
 // Pair table entry (dz, dx0) from the full A, B planes at a representative pair
 // of that depth in the top sponge (row pad - dz): columns pad (dx0 = 0),
@@ -191,27 +185,22 @@ size_t adj_lds_bytes(const Geom &g) {
   return fwd_lds_bytes(g) + (size_t)g.nz * g.nx * sizeof(float);
 }
 
-// Paired kernels (even plane width): [pair table | illumination (adjoint) | guard | plane | guard],
-// and for the adjoint after them its history ring and residual row (kRingBytes).
-// The illumination sits right after the table, so its byte offsets (bits 16-30 of a
-// model pair's descriptor) are relative to an address the LDS instruction adds as
-// an immediate.
+// Strip kernels: [pair table | illumination (adjoint) | guard | plane | guard].
+// The table sits at LDS address 0, so a pair's table entry is read at the byte
+// offset its descriptor holds, with no base add.
 constexpr int pair_guard(int W) { return 2 * W + 2; }
 constexpr int pair_table_floats(int pad) { return (pad + 1) * (pad + 1) * 4; }
-size_t fwd_pair_lds_bytes(const Geom &g) {
+size_t fwd_strip_lds_bytes(const Geom &g) {
   return (size_t)(pair_table_floats(g.pad) + g.H * g.W + 2 * pair_guard(g.W)) * sizeof(float);
+}
+size_t adj_strip_lds_bytes(const Geom &g) {
+  return (size_t)(pair_table_floats(g.pad) + g.nz * g.nx + g.H * g.W + 2 * pair_guard(g.W)) * sizeof(float);
 }
 // Barrier for loops whose cross-wave traffic is LDS only: this wave's LDS
 // accesses complete, then s_barrier.  Unlike __syncthreads() it does not wait
-// for outstanding buffer loads, including direct-to-LDS loads whose data only
-// the issuing wave reads.
+// for outstanding buffer loads and stores.
 __device__ __forceinline__ void lds_barrier() {
   asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
-}
-constexpr int kRingBytes = 3 * 8 * 1024 + 288;  // history ring + residual row of the paired adjoint
-size_t adj_pair_lds_bytes(const Geom &g) {
-  return (size_t)(pair_table_floats(g.pad) + g.nz * g.nx + g.H * g.W + 2 * pair_guard(g.W)) * sizeof(float) +
-         kRingBytes;
 }
 
 // NZ, NX, PAD > 0 and ROW >= 0 fix the geometry at compile time (0 / -1: runtime).
@@ -678,276 +667,6 @@ __global__ __launch_bounds__(kAdjThreads) void adj_resident_kernel(Geom g, int t
   for (int k = t; k < nzx; k += kAdjThreads) illumg[prop * nzx + k] = il[k];
 }
 
-// Paired adjoint for planes of even width (the 150 x 150 plane): thread t owns
-// the cell pairs (2q, 2q+1), q = t + 512 j.  Seven aligned 8-byte LDS reads per
-// pair give both cells' lap_adjoint taps and one 16-byte read their sponge
-// coefficients; the Laplacian history arrives four pairs ahead through a ring
-// of direct-to-LDS loads (the waits for it counted by hand, since the compiler
-// tracks no data that such a load writes), the residual row one step ahead the
-// same way, and c2 two cells per buffer instruction two pairs ahead into
-// registers; within the step loop the barriers wait for LDS accesses only, so
-// ring loads stay in flight across them; the illumination of a model pair is one 8-byte
-// read-modify-write at the address its descriptor holds.  Per cell the
-// arithmetic and its order are those of adj_resident_kernel.  Interior, model
-// and receiver-row ranges all start on even columns, so both cells of a pair
-// share those flags.
-//
-// C2A = true: `c2` holds the plane c2 * A (rounded once, zero on the two-cell
-// rim, made by c2a_kernel), the first product of q = ((c2 A) m1) / 12, so the
-// q update needs neither the sponge table nor the interior flag.  C2A = false:
-// `c2` is c2 itself and A comes from the table.
-template <int KP, int NZ, int NX, int PAD, int ROW, bool C2A>
-__global__ __launch_bounds__(kAdjThreads) void adj_pair_kernel(Geom g, int t0, int t1, const float *__restrict__ c2,
-                                                               const float *__restrict__ A,
-                                                               const float *__restrict__ B,
-                                                               const float *__restrict__ w,
-                                                               const float *__restrict__ sd,
-                                                               const float *__restrict__ res,
-                                                               const float *__restrict__ hist, float *__restrict__ m1g,
-                                                               float *__restrict__ m2g, float *__restrict__ gaccg,
-                                                               float *__restrict__ illumg,
-                                                               float *__restrict__ src_grad) {
-  extern __shared__ float lds[];
-  constexpr int nz = NZ, nx = NX, pad = PAD, row = ROW;
-  constexpr int H = nz + 2 * pad, W = nx + 2 * pad;
-  constexpr int HW = H * W, NP = HW / 2;
-  static_assert(W % 2 == 0 && nx % 2 == 0 && pad % 2 == 0, "pairs need even widths");
-  static_assert((pad + 1) * (pad + 1) * 16 <= (int)kPTabMask + 16, "pair-table offsets must fit the descriptor");
-  static_assert(nz * nx * 4 <= (int)kPIlMask + 1, "illumination offsets must fit the descriptor");
-  constexpr int nzx = nz * nx;
-  constexpr int guard = pair_guard(W), tabf = pair_table_floats(pad);
-  float4 *tab = reinterpret_cast<float4 *>(lds);  // at LDS address 0: lookups need no base add
-  float *il = lds + tabf;
-  constexpr uint32_t kIlAddr = (uint32_t)tabf * 4;  // LDS byte address of the illumination
-  float *Q = il + nzx + guard;
-  float2 *Q2 = reinterpret_cast<float2 *>(Q);
-  constexpr uint32_t kQAddr = (uint32_t)(tabf + nzx + guard) * 4;  // LDS byte address of Q
-  const int64_t prop = xcd_prop(g);
-  if (prop < 0) return;
-  const int b = (int)(prop / g.ns), s = (int)(prop - (int64_t)b * g.ns);
-  const int t = threadIdx.x;
-  for (int k = t; k < guard; k += kAdjThreads) {
-    Q[k - guard] = 0.0f;
-    Q[HW + k] = 0.0f;
-  }
-  fill_pair_table(tab, A, B, W, pad, 1);
-  for (int k = t; k < nzx; k += kAdjThreads) il[k] = illumg[prop * nzx + k];
-  __syncthreads();
-  uint32_t pk[KP];
-#pragma unroll
-  for (int j = 0; j < KP; ++j) {
-    const int q = t + j * kAdjThreads;
-    pk[j] = q < NP ? pair_desc(q, H, W, nz, nx, pad) : 0u;
-  }
-  const __amdgpu_buffer_rsrc_t srs1 = make_rsrc(m1g + prop * HW, HW);
-  const __amdgpu_buffer_rsrc_t srs2 = make_rsrc(m2g + prop * HW, HW);
-  const __amdgpu_buffer_rsrc_t srsg = make_rsrc(gaccg + prop * HW, HW);
-  const __amdgpu_buffer_rsrc_t crs = make_rsrc(c2 + (int64_t)b * HW, (int64_t)HW);
-  float2 m1[KP], m2[KP], ga[KP];
-#pragma unroll
-  for (int j = 0; j < KP; ++j) {
-    m1[j] = m2[j] = ga[j] = make_float2(0.0f, 0.0f);
-    if (t + j * kAdjThreads < NP) {
-      m1[j] = buf_load2(srs1, t * 8, j * kAdjThreads * 8);
-      m2[j] = buf_load2(srs2, t * 8, j * kAdjThreads * 8);
-      ga[j] = buf_load2(srsg, t * 8, j * kAdjThreads * 8);
-    }
-  }
-#pragma unroll
-  for (int j = 0; j < KP; ++j) {
-    const int q = t + j * kAdjThreads;
-    if (q < NP) {
-      const float2 cv = buf_load2(crs, t * 8, j * kAdjThreads * 8);
-      if (C2A) {
-        Q2[q] = div12x2(cv.x * m1[j].x, cv.y * m1[j].y);
-      } else {
-        const float4 e = pair_coef(pk[j]);
-        const float2 qq = div12x2((cv.x * (0.5f * e.x)) * m1[j].x, (cv.y * (0.5f * e.z)) * m1[j].y);
-        Q2[q] = make_float2(keep_if<0>(qq.x, pk[j]), keep_if<0>(qq.y, pk[j]));
-      }
-    }
-  }
-  constexpr int rec0 = (pad + row) * W + pad, rec1 = rec0 + nx;
-  const int srci = rec0 + g.src[s];
-  const int srcq = srci >> 1;
-  const float sdv = sd[prop];
-  const bool own_src = (srcq % kAdjThreads) == t;
-  const int jsrc = srcq / kAdjThreads;
-  float sg = own_src ? src_grad[prop] : 0.0f;
-  const int nslots = t1 - t0;
-  const __amdgpu_buffer_rsrc_t rrs = make_rsrc(res + prop * g.nt * nx, (int64_t)g.nt * nx);
-  const __amdgpu_buffer_rsrc_t hrs = make_rsrc(hist + prop * nslots * HW, (int64_t)nslots * HW);
-  __syncthreads();
-  // History ring in LDS after Q's upper guard: three double slots of 8 KB, each
-  // holding two consecutive pairs of every thread (per wave 2 x 64 x 8 B).  One
-  // 16-byte direct-to-LDS buffer load per wave moves two pairs (lanes 0-31 the
-  // first, 32-63 the second); pairs are loaded four ahead.  Double pair n (counted
-  // across steps) lives in slot n mod 3, so slot bases rotate by two every step.
-  constexpr uint32_t kRingAddr = (uint32_t)(tabf + nzx + 2 * guard + HW) * 4;
-  static_assert(KP % 2 == 0 && (KP / 2) % 3 == 2, "two pairs per load; slots rotate by two per step");
-  const uint32_t wv = __builtin_amdgcn_readfirstlane((uint32_t)t >> 6);
-  const uint32_t lane = (uint32_t)t & 63u;
-  const uint32_t dma_off = 8u * (wv * 64u) + (lane >> 5) * (8u * kAdjThreads) + (lane & 31u) * 16u;
-  // T_i = slot base + wv * 512: a thread reads its pair at o8 + T_i (+ 512 for the second pair)
-  // (o8 = 8 t = wv * 512 + 8 lane); the wave's DMA target is T_i + wv * 512
-  uint32_t T0 = kRingAddr + wv * 512u, T1 = T0 + 8192u, T2 = T0 + 16384u;
-  auto ring = [&](int i) -> uint32_t { return i == 0 ? T0 : (i == 1 ? T1 : T2); };
-  auto issue = [&](int jn, int s) {  // pairs jn, jn+1 (jn >= KP: of the next step) of time slot s
-    const int p = jn < KP ? jn : jn - KP;
-    __attribute__((address_space(3))) void *dst =
-        (__attribute__((address_space(3))) void *)(uintptr_t)(ring((jn / 2) % 3) + wv * 512u);
-    __builtin_amdgcn_raw_ptr_buffer_load_lds(hrs, dst, 16, dma_off + (uint32_t)p * 8u * kAdjThreads,
-                                             (uint32_t)(s * HW * 4), 0, 0);
-  };
-  // residual row of the next step: 18 lanes of wave 0 move 72 floats into LDS after the ring
-  constexpr uint32_t kResAddr = kRingAddr + 3u * 8192u;
-  auto issue_res = [&](int it_next) {
-    if (wv == 0 && lane < 18u)
-      __builtin_amdgcn_raw_ptr_buffer_load_lds(rrs, (__attribute__((address_space(3))) void *)(uintptr_t)kResAddr, 16,
-                                               lane * 16u, (uint32_t)(it_next * nx * 4), 0, 0);
-  };
-  constexpr int jrec = (rec0 / 2) / kAdjThreads;
-  static_assert((rec1 / 2 - 1) / kAdjThreads == jrec && jrec >= 4, "receiver pairs in one pair slot, after the first");
-  static_assert(nx % 2 == 0 && nx <= 72, "the residual row fits the staging slot");
-  static_assert(rec0 / 2 >= jrec * kAdjThreads && rec1 / 2 - 1 < jrec * kAdjThreads + 64,
-                "receiver pairs belong to wave 0, which stages the residual");
-  issue_res(t1 - 1);
-  constexpr int DC = 4;  // c2 (or c2 A) loads DC pairs ahead (11 measured 0.7 % faster but needs every register)
-  // pair n of the whole run (step-major) sits in slot n mod DC; two steps make a whole number of turns
-  static_assert((2 * KP) % DC == 0, "two steps of pairs must fill the c2 ring a whole number of times");
-  float2 Cb[DC];
-#pragma unroll
-  for (int d = 0; d < DC; ++d) {
-    const bool live = t + d * kAdjThreads < NP;
-    Cb[d] = live ? buf_load2(crs, t * 8, d * kAdjThreads * 8) : make_float2(0.0f, 0.0f);
-  }
-  issue(0, t1 - 1 - t0);
-  issue(2, t1 - 1 - t0);
-  // The time loop runs two steps per iteration with the roles of the two state arrays
-  // swapped in the second, so the new state is written over the oldest one and no
-  // register moves the state along.
-  auto step = [&](float2(&M1)[KP], float2(&M2)[KP], const int it, auto phase) __attribute__((always_inline)) {
-    constexpr int cs = (decltype(phase)::value * KP) % DC;  // c2 ring slot of this step's pair 0
-    const int slot = it - t0;
-    const float wt = w[it];
-    float2 msrc = make_float2(0.0f, 0.0f);  // the new state of the source's pair (its owner's lane)
-    uint32_t o8 = 8u * t;  // the thread's byte offset within a row of pairs
-    uint32_t rb[3] = {o8 + T0, o8 + T1, o8 + T2};  // this thread's read addresses in the three ring slots
-    float2 tp[2][7];
-    // the seven lap_adjoint taps of pair jj: immediate offsets from one base address two rows up
-    auto load_taps = [&](int jj, float2(&d)[7]) {
-      uint32_t a = o8 + (kQAddr + 8u * (uint32_t)(jj * kAdjThreads - W));
-      asm volatile("" : "+v"(a));
-      d[0] = lds_ld2(a);
-      d[1] = lds_ld2(a + 4 * W);
-      d[2] = lds_ld2(a + 8 * W - 8);
-      d[3] = lds_ld2(a + 8 * W);
-      d[4] = lds_ld2(a + 8 * W + 8);
-      d[5] = lds_ld2(a + 12 * W);
-      d[6] = lds_ld2(a + 16 * W);
-    };
-    load_taps(0, tp[0]);
-    opaque(pk);
-#pragma unroll
-    for (int j = 0; j < KP; ++j) {
-      asm volatile("" : "+v"(o8), "+v"(pk[j]));  // one pair at a time
-      if (j % 2 == 0) {
-        issue(j + 4, j + 4 < KP ? slot : (slot > 0 ? slot - 1 : slot));
-        // the load of pairs j, j+1 is complete once at most the two issued after it are outstanding
-        asm volatile("s_waitcnt vmcnt(2)" ::: "memory");
-      }
-      const float2 L = lds_ld2(rb[(j / 2) % 3] + (uint32_t)(j & 1) * 512u);
-      // the next pair's taps are read before this pair computes (one pair of reads in flight)
-      if (j + 1 < KP && t + (j + 1) * kAdjThreads < NP) load_taps(j + 1, tp[(j + 1) & 1]);
-      if (t + j * kAdjThreads < NP) {
-        const int q = (int)(o8 >> 3) + j * kAdjThreads;
-        const float2 u2m = tp[j & 1][0], um = tp[j & 1][1], r0 = tp[j & 1][2], r1 = tp[j & 1][3];
-        const float2 r2 = tp[j & 1][4], up = tp[j & 1][5], u2p = tp[j & 1][6];
-        const float lt0 = lapt_sum(r1.x, u2p.x, up.x, um.x, u2m.x, r2.x, r1.y, r0.y, r0.x);
-        const float lt1 = lapt_sum(r1.y, u2p.y, up.y, um.y, u2m.y, r2.y, r2.x, r1.x, r0.y);
-        const float4 e = pair_coef(pk[j]);  // (2A0, A0 B0, 2A1, A1 B1)
-        float mu0 = e.x * M1[j].x;
-        float mu1 = e.z * M1[j].y;
-        mu0 = mu0 + lt0;
-        mu1 = mu1 + lt1;
-        mu0 = mu0 - e.y * M2[j].x;
-        mu1 = mu1 - e.w * M2[j].y;
-        if (j == jrec && q >= rec0 / 2 && q < rec1 / 2) {
-          // staged by the previous step; complete since the ring loads issued after it are
-          const float2 rv = lds_ld2(kResAddr + (uint32_t)(2 * q - rec0) * 4u);
-          mu0 = mu0 + rv.x;
-          mu1 = mu1 + rv.y;
-        }
-        ga[j].x = ga[j].x + ((0.5f * e.x) * mu0) * L.x;  // A = 0.5 (2A), exact
-        ga[j].y = ga[j].y + ((0.5f * e.z) * mu1) * L.y;
-        if (j == jsrc) msrc = make_float2(mu0, mu1);  // the source's pair slot (uniform): two selects
-        if ((int32_t)pk[j] < 0) {  // model pair: its illumination address is in the descriptor
-          const uint32_t ia = kIlAddr + ((pk[j] >> 16) & kPIlMask);
-          float2 v = lds_ld2(ia);
-          v.x = v.x + L.x * L.x;
-          v.y = v.y + L.y * L.y;
-          lds_st2(ia, v);
-        }
-        M2[j] = make_float2(mu0, mu1);  // the new state; M1 (the old state) becomes the next step's m2
-        asm volatile("" : "+v"(ga[j].x), "+v"(ga[j].y));
-      }
-      // the ring slot just read is refilled two pairs later: its read must be complete
-      asm volatile("" ::"v"(L.x), "v"(L.y) : "memory");
-    }
-    if (own_src) sg = sg + (((srci & 1) ? msrc.y : msrc.x) * wt) * sdv;  // gpu.py:182, in step order
-    lds_barrier();
-    int off = t;
-    if (it > t0) issue_res(it - 1);
-    asm volatile("" : "+v"(off));  // keep per-cell descriptors from being hoisted out of the time loop
-    opaque(pk);
-#pragma unroll
-    for (int j = 0; j < KP; ++j) {
-      asm volatile("" : "+v"(pk[j]));
-      const float2 cv = Cb[(cs + j) % DC];
-      {
-        // c2 is the same every step: after the last pair, load the first pairs of the next step
-        const int jn = j + DC < KP ? j + DC : j + DC - KP;
-        if (t + jn * kAdjThreads < NP) Cb[(cs + j) % DC] = buf_load2(crs, off * 8, jn * kAdjThreads * 8);
-      }
-      if (t + j * kAdjThreads < NP) {
-        if (C2A) {
-          Q2[off + j * kAdjThreads] = div12x2(cv.x * M2[j].x, cv.y * M2[j].y);
-        } else {
-          const float4 e = pair_coef(pk[j]);
-          const float2 qq = div12x2((cv.x * (0.5f * e.x)) * M2[j].x, (cv.y * (0.5f * e.z)) * M2[j].y);
-          // both cells of a pair share the interior flag (bit 0)
-          Q2[off + j * kAdjThreads] = make_float2(keep_if<0>(qq.x, pk[j]), keep_if<0>(qq.y, pk[j]));
-        }
-      }
-    }
-    lds_barrier();
-    {
-      const uint32_t x = T2;  // next step: double pair n + 11 is in slot (n + 2) mod 3
-      T2 = T1;
-      T1 = T0;
-      T0 = x;
-    }
-  };
-  int it = t1 - 1;
-  for (; it - 1 >= t0; it -= 2) {
-    step(m1, m2, it, std::integral_constant<int, 0>());
-    step(m2, m1, it - 1, std::integral_constant<int, 1>());
-  }
-  const bool odd = it >= t0;  // one step left: the current state is then in m2
-  if (odd) step(m1, m2, it, std::integral_constant<int, 0>());
-  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // the ring loads of the last step land before exit
-#pragma unroll
-  for (int j = 0; j < KP; ++j) {
-    if (t + j * kAdjThreads < NP) {
-      buf_store2(odd ? m2[j] : m1[j], srs1, t * 8, j * kAdjThreads * 8);
-      buf_store2(odd ? m1[j] : m2[j], srs2, t * 8, j * kAdjThreads * 8);
-      buf_store2(ga[j], srsg, t * 8, j * kAdjThreads * 8);
-    }
-  }
-  if (own_src) src_grad[prop] = sg;
-  for (int k = t; k < nzx; k += kAdjThreads) illumg[prop * nzx + k] = il[k];
-}
-
 // Column-strip adjoint for the fixed 150 x 150 plane with the c2 * A plane
 // (c2a_kernel: zero on the two-cell rim, so q is zero there, as lap_adjoint
 // reads only p[2:-2, 2:-2], gpu.py:41).  The strip layout of fwd_strip_kernel:
@@ -967,7 +686,11 @@ __global__ __launch_bounds__(kAdjThreads) void adj_pair_kernel(Geom g, int t0, i
 // eight-row ring from the L2-resident plane.  Lanes t >= 750 mirror thread 749
 // (same reads, same state); they write nothing but phase 2's q, whose values
 // equal thread 749's.
-template <int NZ, int NX, int PAD, int ROW>
+//
+// C2A = false (callers without the c2 * A plane): `c2a` is c2 itself; phase 2
+// forms c2 * A with A = 0.5 * 2A from the row's sponge entry and zeroes q on
+// the rim by the interior flag.
+template <int NZ, int NX, int PAD, int ROW, bool C2A>
 __global__ __launch_bounds__(kStripThreads) void adj_strip_kernel(Geom g, int t0, int t1,
                                                                   const float *__restrict__ c2a,
                                                                   const float *__restrict__ A,
@@ -1015,7 +738,7 @@ __global__ __launch_bounds__(kStripThreads) void adj_strip_kernel(Geom g, int t0
 
   float2 m1[R], m2[R], ga[R];
   uint32_t tb[R];
-  bool mdl[R];
+  bool mdl[R], inner[R];
   const bool colmodel = act && c >= pad / 2 && c < (pad + nx) / 2;
   const float2 *m1p = reinterpret_cast<const float2 *>(m1g + prop * HW);
   const float2 *m2p = reinterpret_cast<const float2 *>(m2g + prop * HW);
@@ -1027,12 +750,23 @@ __global__ __launch_bounds__(kStripThreads) void adj_strip_kernel(Geom g, int t0
     m1[k] = m1p[q];
     m2[k] = m2p[q];
     ga[k] = gap[q];
-    if (act) {
-      const float2 cv = cap[q];
-      Q2[q] = div12x2(cv.x * m1[k].x, cv.y * m1[k].y);
-    }
-    tb[k] = pair_desc(q, H, W, nz, nx, pad) & kPTabMask;
+    const uint32_t d = pair_desc(q, H, W, nz, nx, pad);
+    tb[k] = d & kPTabMask;
+    inner[k] = (d & kPInterior) != 0;
     mdl[k] = colmodel && r0 + k >= pad && r0 + k < pad + nz;
+  }
+  // q = ((c2 A) m) / 12 of row k from c2 A (C2A) or c2 and the row's (2A) entries
+  auto qrow = [&](int k, float2 cv, float2 m) __attribute__((always_inline)) {
+    if (C2A) return div12x2(cv.x * m.x, cv.y * m.y);
+    const float4 e = lds_ld4(tb[k]);
+    const float2 qq = div12x2((cv.x * (0.5f * e.x)) * m.x, (cv.y * (0.5f * e.z)) * m.y);
+    return inner[k] ? qq : make_float2(0.0f, 0.0f);
+  };
+  __syncthreads();  // the sponge table, for qrow
+#pragma unroll
+  for (int k = 0; k < R; ++k) {
+    const int q = (r0 + k) * NC + c;
+    if (act) Q2[q] = qrow(k, cap[q], m1[k]);
   }
   const int srcx = pad + g.src[s];
   const bool own_src = act && strip == RS && c == (srcx >> 1);
@@ -1132,7 +866,7 @@ __global__ __launch_bounds__(kStripThreads) void adj_strip_kernel(Geom g, int t0
     for (int k = 0; k < R; ++k) {
       const float2 cv = cb[k % DC];
       if (k + DC < R) cb[k % DC] = buf_load2(crs, hv, (k + DC) * (int)kRow);
-      lds_st2(aa + (uint32_t)(k + 2) * kRow, div12x2(cv.x * M2[k].x, cv.y * M2[k].y));
+      lds_st2(aa + (uint32_t)(k + 2) * kRow, qrow(k, cv, M2[k]));
     }
     if (it - 1 >= t0) load_step(it - 1);
     lds_barrier();
@@ -1158,10 +892,6 @@ __global__ __launch_bounds__(kStripThreads) void adj_strip_kernel(Geom g, int t0
   }
   if (own_src) src_grad[prop] = sg;
   for (int k = t; k < nzx; k += kStripThreads) illumg[prop * nzx + k] = il[k];
-}
-
-size_t adj_strip_lds_bytes(const Geom &g) {
-  return (size_t)(pair_table_floats(g.pad) + g.nz * g.nx + g.H * g.W + 2 * pair_guard(g.W)) * sizeof(float);
 }
 
 // c2a = c2 * A on the interior cells of every plane of c2 (nb, H, W), zero on the
@@ -1233,14 +963,14 @@ bool resident_fits(const Geom &g) {
   if (slots_for(g, kFwdThreads) > kFwdMaxSlots || slots_for(g, kAdjThreads) > kAdjMaxSlots) return false;
   if ((g.pad + 1) * (g.pad + 1) > (int)kIdxMask + 1 || g.nz * g.nx > (int)kIdxMask + 1) return false;
   if ((int64_t)g.nt * g.H * g.W > 0x1fffffff) return false;  // history slots addressed with 32-bit byte offsets
-  const size_t need = is_openfwi(g) ? std::max(adj_pair_lds_bytes(g), fwd_pair_lds_bytes(g)) : adj_lds_bytes(g);
+  const size_t need = is_openfwi(g) ? std::max(adj_strip_lds_bytes(g), fwd_strip_lds_bytes(g)) : adj_lds_bytes(g);
   return need <= (size_t)device_lds_limit();
 }
 
 int forward_resident(const Geom &g, int t0, int t1, const float *c2, const float *A, const float *B, const float *w,
                      const float *amp, float *p0, float *p1, float *gathers, float *hist, float *ckpt,
                      int ckpt_every, hipStream_t st) {
-  const size_t lds = is_openfwi(g) ? fwd_pair_lds_bytes(g) : fwd_lds_bytes(g);
+  const size_t lds = is_openfwi(g) ? fwd_strip_lds_bytes(g) : fwd_lds_bytes(g);
   return dispatch_fwd(g, [&](auto kc, auto fixed) -> int {
     constexpr bool F = decltype(fixed)::value;
     // the fixed 150 x 150 plane runs the paired kernel (11 pairs per thread)
@@ -1259,20 +989,19 @@ int forward_resident(const Geom &g, int t0, int t1, const float *c2, const float
 int adjoint_resident(const Geom &g, int t0, int t1, const float *c2, const float *A, const float *B, const float *w,
                      const float *sd, const float *res, const float *hist, float *m1, float *m2, float *gacc,
                      float *illum, float *src_grad, hipStream_t st, const float *c2a) {
-  const bool pre = is_openfwi(g) && c2a != nullptr;
-  const size_t lds = pre ? adj_strip_lds_bytes(g) : (is_openfwi(g) ? adj_pair_lds_bytes(g) : adj_lds_bytes(g));
+  const bool pre = c2a != nullptr;
+  const size_t lds = is_openfwi(g) ? adj_strip_lds_bytes(g) : adj_lds_bytes(g);
   return dispatch_adj(g, [&](auto kc, auto fixed) -> int {
     constexpr bool F = decltype(fixed)::value;
-    // the fixed 150 x 150 plane runs the column-strip kernel when the caller has the
-    // c2 * A plane, else the paired kernel (22 pairs per thread)
-    constexpr int KP = kAdjPairs150;
-    auto kern = F ? (pre ? adj_strip_kernel<70, 70, 40, 1> : adj_pair_kernel<KP, 70, 70, 40, 1, false>)
+    // the fixed 150 x 150 plane runs the column-strip kernel, with the c2 * A plane
+    // when the caller has one
+    auto kern = F ? (pre ? adj_strip_kernel<70, 70, 40, 1, true> : adj_strip_kernel<70, 70, 40, 1, false>)
                   : adj_resident_kernel<decltype(kc)::value, F ? 70 : 0, F ? 70 : 0, F ? 40 : 0, F ? 1 : -1>;
     hipError_t e = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return hip_status(e, "adjoint_resident attribute");
     if (F && !no_static_lds((const void *)kern)) return hip_status(hipErrorInvalidValue, "adjoint_resident static LDS");
-    kern<<<dim3((unsigned)xcd_blocks((int64_t)g.nb * g.ns)), dim3(pre ? kStripThreads : kAdjThreads), lds, st>>>(
-        g, t0, t1, pre ? c2a : c2, A, B, w, sd, res, hist, m1, m2, gacc, illum, src_grad);
+    kern<<<dim3((unsigned)xcd_blocks((int64_t)g.nb * g.ns)), dim3(F ? kStripThreads : kAdjThreads), lds, st>>>(
+        g, t0, t1, F && pre ? c2a : c2, A, B, w, sd, res, hist, m1, m2, gacc, illum, src_grad);
     return hip_status(hipGetLastError(), "adjoint_resident launch");
   });
 }

@@ -433,3 +433,38 @@ def test_pinned_host_observations_equal_device_observations(history):
     assert torch.equal(img_h, img_d) and torch.equal(ill_h, ill_d)
     with pytest.raises(ValueError, match='pinned'):
         sim.images_device(prep, obs.cpu(), 3, vm)
+
+
+def test_public_resident_adjoint_equals_stepwise():
+    """The adjoint entry point (ss_adjoint_f32) on the 150 x 150 plane with the
+    resident kernels and no c2*A plane (the column-strip adjoint then forms c2*A
+    from its sponge table) equals the stepwise kernels bit for bit: imaging sum,
+    illumination, source term and final state, over the whole record and over two
+    segments of odd length that carry the state between calls (gpu.py:161-184)."""
+    case('public resident adjoint vs stepwise, whole record and two segments')
+    sim = gpu.Sim(device='cuda')
+    g = sim.geom
+    models = synth.mixed(1, seed=11)
+    prep = sim._prepare(np.repeat(models, 2, 0))
+    nb, H, W = prep['nb'], sim.H, sim.W
+    geom = sim._geom(nb)
+    z = lambda *s: torch.zeros(*s, device=DEV)
+    p0, p1 = z(nb, g.ns, H, W), z(nb, g.ns, H, W)
+    hist = torch.empty(nb, g.ns, g.nt, H, W, device=DEV)
+    sim._forward(geom, 0, g.nt, prep['c2'], prep['amp'], p0, p1, hist=hist, impl=_lib.IMPL_STEPWISE)
+    gen = torch.Generator(device=DEV).manual_seed(5)
+    res = torch.randn(nb, g.ns, g.nt, g.nx, generator=gen, device=DEV)
+
+    def run(impl, cuts):
+        m1, m2, gacc = z(nb, g.ns, H, W), z(nb, g.ns, H, W), z(nb, g.ns, H, W)
+        illum, sg = z(nb, g.ns, g.nz, g.nx), z(nb, g.ns)
+        for t0, t1 in cuts:  # reverse time: later segments first
+            seg = hist[:, :, t0:t1].contiguous()
+            sim._adjoint(geom, t0, t1, prep['c2'], prep['sd'], res, seg, m1, m2, gacc, illum, sg, impl=impl)
+        return m1, m2, gacc, illum, sg
+
+    want = run(_lib.IMPL_STEPWISE, [(0, g.nt)])
+    for cuts in ([(0, g.nt)], [(377, g.nt), (0, 377)]):
+        got = run(_lib.IMPL_RESIDENT, cuts)
+        for name, a, b in zip(('m1', 'm2', 'imaging sum', 'illumination', 'source term'), got, want):
+            assert torch.equal(a, b), f'{name}, segments {cuts}'
