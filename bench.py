@@ -72,7 +72,9 @@ def roofline(kernel, kernel_ms, alg_bytes):
            'note': 'VALU lane-ops (64 per instruction) per second against 256 CUs x 64 lanes x 2.4 GHz; '
                    'traffic = measured HBM bytes per launch (FETCH_SIZE x2 + WRITE_SIZE, scaled by a 1 GiB '
                    'copy); the 16 B/cell-update algorithmic rate exceeds the HBM peak because the wavefield '
-                   'stays in LDS and registers'}
+                   'stays in LDS and registers.  The launches run at the board power cap (board_power), '
+                   'which holds the clock below 2.4 GHz: counter_run gives the clock and the VALU-busy '
+                   'fraction at that clock'}
     rec = counters()
     k = rec and rec['kernels'].get(kernel)
     if not k:
@@ -92,6 +94,51 @@ def roofline(kernel, kernel_ms, alg_bytes):
 
 def median_rate(units, seconds):
     return units / float(np.median(seconds))
+
+
+class BoardPower:
+    """Samples the GPU's board power (hwmon power1_input, microwatts) every 10 ms on a
+    host thread while the context is open; `summary()` gives the mean and maximum in
+    watts and the board's power cap when the driver exposes them, else None."""
+
+    def __init__(self):
+        import glob
+        files = sorted(glob.glob('/sys/class/drm/card*/device/hwmon/hwmon*/power1_input'))
+        self.path = files[0] if files else None
+        self.samples, self._stop, self._thread = [], False, None
+
+    def _read(self, path):
+        try:
+            with open(path) as f:
+                return float(f.read()) / 1e6
+        except (OSError, ValueError):
+            return None
+
+    def __enter__(self):
+        import threading
+        if self.path and self._read(self.path) is not None:
+            def loop():
+                while not self._stop:
+                    v = self._read(self.path)
+                    if v is not None:
+                        self.samples.append(v)
+                    time.sleep(0.01)
+            self._thread = threading.Thread(target=loop, daemon=True)
+            self._thread.start()
+        return self
+
+    def __exit__(self, *exc):
+        self._stop = True
+        if self._thread:
+            self._thread.join()
+
+    def summary(self):
+        if len(self.samples) < 4:
+            return None
+        s = np.asarray(self.samples[len(self.samples) // 4:])   # skip the ramp at the start
+        cap = self._read(self.path.replace('power1_input', 'power1_cap'))
+        return {'mean_W': float(s.mean()), 'max_W': float(s.max()), 'samples': int(len(s)),
+                'cap_W': cap, 'source': 'hwmon power1_input sampled every 10 ms during the timed steps'}
 
 
 # ------------------------------------------------------------------ headline
@@ -194,11 +241,12 @@ def run_batched(args):
     sim.timing, sim.timing_log = True, []
     step_s = []
     t_all = time.perf_counter()
-    for _ in range(args.steps):
-        t0 = time.perf_counter()
-        img, illum = sim.images_device(prep, obs, 3, vm)
-        torch.cuda.synchronize()
-        step_s.append(time.perf_counter() - t0)
+    with BoardPower() as power:
+        for _ in range(args.steps):
+            t0 = time.perf_counter()
+            img, illum = sim.images_device(prep, obs, 3, vm)
+            torch.cuda.synchronize()
+            step_s.append(time.perf_counter() - t0)
     wall = time.perf_counter() - t_all
     sim.timing = False
     per_kernel = {}
@@ -223,6 +271,7 @@ def run_batched(args):
                    'step_ms': [s * 1e3 for s in step_s], 'mean_rate': shot_images * args.steps / wall},
         'roofline': roofline(dominant, avg[dominant], alg_bytes),
         'kernel_ms': avg,
+        'board_power': power.summary(),
     }
     prot_s, pimg, pill = protocol_runs(sim, bgs, obs_host, v_mute)
     result['protocol'] = {

@@ -37,3 +37,26 @@ def test_cpu_baseline_is_median_of_three_timed_runs():
     np.testing.assert_array_equal(img, want)       # thirds of the sample, concatenated in order
     e = np.linalg.norm(img.astype(np.float64) - img64) / np.linalg.norm(img64)
     assert 0 < e < 1e-4                             # FP32 against FP64 rounding, not a different result
+
+
+def test_board_power_sampler(tmp_path):
+    """bench.BoardPower reads a hwmon power file (microwatts) on a host thread while
+    open and reports mean and maximum watts after the first quarter of samples, the
+    power cap beside it; without a readable file it reports nothing."""
+    import time
+    sys.path.insert(0, str(ROOT))
+    import bench
+    f = tmp_path / 'power1_input'
+    f.write_text('1250000000\n')
+    (tmp_path / 'power1_cap').write_text('1400000000\n')
+    p = bench.BoardPower()
+    p.path = str(f)
+    with p:
+        time.sleep(0.15)
+    s = p.summary()
+    assert s['mean_W'] == s['max_W'] == 1250.0 and s['cap_W'] == 1400.0 and s['samples'] >= 3
+    q = bench.BoardPower()
+    q.path = str(tmp_path / 'missing')
+    with q:
+        time.sleep(0.05)
+    assert q.summary() is None
