@@ -89,6 +89,14 @@ __device__ __forceinline__ float keep_if(float v, uint32_t d) {
   return __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, v) & m);
 }
 
+// Barrier for loops whose cross-wave traffic is LDS only: this wave's LDS
+// accesses complete, then s_barrier.  Unlike __syncthreads() it does not wait
+// for outstanding buffer loads and stores (history streams, receiver samples),
+// which later launches or later loop iterations consume.
+__device__ __forceinline__ void lds_barrier() {
+  asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
 // floor(n / d) with magic = 0xffffffff / d + 1, exact whenever n * d < 2^32:
 // n * magic / 2^32 exceeds n / d by less than n / 2^32 < 1/d, which is no more
 // than the distance from n / d up to the next integer.

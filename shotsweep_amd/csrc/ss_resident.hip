@@ -196,12 +196,6 @@ size_t fwd_strip_lds_bytes(const Geom &g) {
 size_t adj_strip_lds_bytes(const Geom &g) {
   return (size_t)(pair_table_floats(g.pad) + g.nz * g.nx + g.H * g.W + 2 * pair_guard(g.W)) * sizeof(float);
 }
-// Barrier for loops whose cross-wave traffic is LDS only: this wave's LDS
-// accesses complete, then s_barrier.  Unlike __syncthreads() it does not wait
-// for outstanding buffer loads and stores.
-__device__ __forceinline__ void lds_barrier() {
-  asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
-}
 
 // NZ, NX, PAD > 0 and ROW >= 0 fix the geometry at compile time (0 / -1: runtime).
 template <int K, int NZ, int NX, int PAD, int ROW>

@@ -235,7 +235,7 @@ __global__ __launch_bounds__(kThreads) void sweep_fwd_kernel(SArgs a) {
         pn[j] = c;
       }
     }
-    __syncthreads();
+    lds_barrier();  // the history and receiver accesses stay in flight
     float *tmp = Pc;
     Pc = Pn;
     Pn = tmp;
@@ -352,7 +352,7 @@ __global__ __launch_bounds__(kThreads) void sweep_adj_kernel(SArgs a) {
         Qn[e] = (d & kInt) ? div12(ca[j] * mu) : 0.0f;
       }
     }
-    __syncthreads();
+    lds_barrier();  // the history and receiver accesses stay in flight
     float *tmp = Qc;
     Qc = Qn;
     Qn = tmp;
@@ -496,7 +496,7 @@ __global__ __launch_bounds__(kThreads) void sweep_fwd_pair_kernel(SArgs a) {
       float *ps = reinterpret_cast<float *>(N2 + t + src_j * kThreads) + src_hi;
       *ps = *ps + wsrc;
     }
-    __syncthreads();
+    lds_barrier();  // the history and receiver accesses stay in flight
     float *tmp = Pc;
     Pc = Pn;
     Pn = tmp;
@@ -652,7 +652,7 @@ __global__ __launch_bounds__(kThreads) void sweep_adj_pair_kernel(SArgs a) {
       }
     }
     if (src_j >= 0 && t == src_t) sg = sg + ((src_hi ? msrc.y : msrc.x) * wt) * sdv;  // gpu.py:182, step order
-    __syncthreads();
+    lds_barrier();  // the history and receiver accesses stay in flight
     float *tmp = Qc;
     Qc = Qn;
     Qn = tmp;
