@@ -52,9 +52,12 @@ int velocity_update(int B, int64_t n, const float *v, const float *d, const floa
 // Resident FP32 kernels (plane held in LDS, one workgroup per propagation).
 int64_t resident_max_cells();
 bool resident_fits(const Geom &g);
+// exact_rim = false: the 150 x 150 plane's history is left unspecified on the two-cell
+// rim (only for the native driver, whose adjoint_resident with the c2 * A plane never
+// uses it); every other output is unchanged.
 int forward_resident(const Geom &g, int t0, int t1, const float *c2, const float *A, const float *B, const float *w,
                      const float *amp, float *p0, float *p1, float *gathers, float *hist, float *ckpt,
-                     int ckpt_every, hipStream_t st);
+                     int ckpt_every, hipStream_t st, bool exact_rim = true);
 // First-break picks (onset index, peak |g|) per trace of (ngathers, nt, nx) gathers.
 template <typename T>
 int first_breaks(int64_t ngathers, int nt, int nx, const T *obs, double frac, int *onset, T *peak, hipStream_t st);

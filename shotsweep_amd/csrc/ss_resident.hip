@@ -114,7 +114,8 @@ __device__ __forceinline__ uint32_t pair_desc(int q, int H, int W, int nz, int n
 // Pair table entry (dz, dx0) from the full A, B planes at a representative pair
 // of that depth in the top sponge (row pad - dz): columns pad (dx0 = 0),
 // pad - dx0 (dx0 even, left) or W - 1 - pad + dx0 (dx0 odd, right).
-// mode 0: (A0, B0, A1, B1); mode 1: (2A0, A0 B0, 2A1, A1 B1).
+// mode 0: (A0, B0, A1, B1); mode 1: (2A0, A0 B0, 2A1, A1 B1); mode 2: as mode 1
+// but zero for the pairs on the two-cell rim (dz or dx0 at least pad - 1).
 __device__ void fill_pair_table(float4 *tab, const float *A, const float *B, int W, int pad, int mode) {
   const int n = (pad + 1) * (pad + 1);
   for (int k = threadIdx.x; k < n; k += blockDim.x) {
@@ -123,6 +124,7 @@ __device__ void fill_pair_table(float4 *tab, const float *A, const float *B, int
     const int c = (pad - dz) * W + x0;
     const float a0 = A[c], b0 = B[c], a1 = A[c + 1], b1 = B[c + 1];
     tab[k] = mode ? make_float4(2.0f * a0, a0 * b0, 2.0f * a1, a1 * b1) : make_float4(a0, b0, a1, b1);
+    if (mode == 2 && (dz >= pad - 1 || dx0 >= pad - 1)) tab[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   }
 }
 
@@ -344,7 +346,7 @@ constexpr int kStripRows = 15, kStripThreads = 768;
 // One time step of a strip: PC holds p(t) of the strip's rows, PO p(t-1) on entry and
 // p(t+1) on exit.  Only the two rows above and below the strip and the left and right
 // neighbour pairs of every row come from LDS.
-template <int R, int W, int RK>
+template <int R, int W, int RK, bool RIM>
 __device__ __forceinline__ void strip_step(float2 (&PO)[R], float2 (&PC)[R], const float2 (&cc)[R],
                                            const uint32_t (&tb)[R], const bool (&inner)[R], uint32_t a,
                                            __amdgpu_buffer_rsrc_t hrs, int hv, int hoff,
@@ -375,7 +377,7 @@ __device__ __forceinline__ void strip_step(float2 (&PO)[R], float2 (&PC)[R], con
     const float s0 = lap_sum(zm2.x, zm1.x, zc.x, zp1.x, zp2.x, lf.x, lf.y, zc.y, rt.x);
     const float s1 = lap_sum(zm2.y, zm1.y, zc.y, zp1.y, zp2.y, lf.y, zc.x, rt.x, rt.y);
     const float2 Lq = div12x2(s0, s1);
-    const float L0 = inner[k] ? Lq.x : 0.0f, L1 = inner[k] ? Lq.y : 0.0f;
+    const float L0 = !RIM || inner[k] ? Lq.x : 0.0f, L1 = !RIM || inner[k] ? Lq.y : 0.0f;
     PO[k].x = leapfrog(zc.x, PO[k].x, ab.y, cc[k].x, L0, ab.x);
     PO[k].y = leapfrog(zc.y, PO[k].y, ab.w, cc[k].y, L1, ab.z);
     buf_store2(make_float2(L0, L1), hrs, hv, hoff + k * (int)kRow);
@@ -399,7 +401,14 @@ __device__ __forceinline__ void strip_step(float2 (&PO)[R], float2 (&PC)[R], con
   lds_barrier();
 }
 
-template <int NZ, int NX, int PAD, int ROW>
+//
+// RIM = true: the Laplacian history holds zero on the two-cell rim, as gpu.py:35
+// writes it (the forward entry point's contract).  RIM = false (the native
+// driver, whose c2 * A adjoint zeroes the rim's sponge entries and so never uses
+// the rim's history): no per-pair select; c2 is zero on the rim pairs instead, so
+// ((2 p1 - B p0) + 0 * L) A keeps the rim's wavefield at +0 as the reference's
+// zero Laplacian does, whatever finite value the rim's L has.
+template <int NZ, int NX, int PAD, int ROW, bool RIM>
 __global__ __launch_bounds__(kStripThreads) void fwd_strip_kernel(Geom g, int t0, int t1, const float *__restrict__ c2,
                                                                   const float *__restrict__ A,
                                                                   const float *__restrict__ B,
@@ -446,11 +455,11 @@ __global__ __launch_bounds__(kStripThreads) void fwd_strip_kernel(Geom g, int t0
     const int q = (r0 + k) * NC + c;
     Pa[k] = p0p[q];
     Pb[k] = p1p[q];
-    cc[k] = c2b[q];
-    if (act) P2[q] = Pb[k];
     const uint32_t d = pair_desc(q, H, W, nz, nx, pad);
     tb[k] = d & kPTabMask;
     inner[k] = (d & kPInterior) != 0;
+    cc[k] = RIM || inner[k] ? c2b[q] : make_float2(0.0f, 0.0f);
+    if (act) P2[q] = Pb[k];
   }
   const int srcx = pad + g.src[s];
   const bool src_owner = act && strip == RS && c == (srcx >> 1);
@@ -481,17 +490,17 @@ __global__ __launch_bounds__(kStripThreads) void fwd_strip_kernel(Geom g, int t0
   int it = t0;
   for (; it + 1 < t1; it += 2) {
     ckpt_store(it - t0, Pa, Pb);
-    strip_step<R, W, RK>(Pa, Pb, cc, tb, inner, a, hrs, hv, (it - t0) * HW * 4, src_owner, src_hi,
-                         w[it] * ampv, rec_thread, grs, gv, it * nx * 4, act);
+    strip_step<R, W, RK, RIM>(Pa, Pb, cc, tb, inner, a, hrs, hv, (it - t0) * HW * 4, src_owner, src_hi,
+                              w[it] * ampv, rec_thread, grs, gv, it * nx * 4, act);
     ckpt_store(it + 1 - t0, Pb, Pa);
-    strip_step<R, W, RK>(Pb, Pa, cc, tb, inner, a, hrs, hv, (it + 1 - t0) * HW * 4, src_owner, src_hi,
-                         w[it + 1] * ampv, rec_thread, grs, gv, (it + 1) * nx * 4, act);
+    strip_step<R, W, RK, RIM>(Pb, Pa, cc, tb, inner, a, hrs, hv, (it + 1 - t0) * HW * 4, src_owner, src_hi,
+                              w[it + 1] * ampv, rec_thread, grs, gv, (it + 1) * nx * 4, act);
   }
   const bool odd = it < t1;
   if (odd) {
     ckpt_store(it - t0, Pa, Pb);
-    strip_step<R, W, RK>(Pa, Pb, cc, tb, inner, a, hrs, hv, (it - t0) * HW * 4, src_owner, src_hi,
-                         w[it] * ampv, rec_thread, grs, gv, it * nx * 4, act);
+    strip_step<R, W, RK, RIM>(Pa, Pb, cc, tb, inner, a, hrs, hv, (it - t0) * HW * 4, src_owner, src_hi,
+                              w[it] * ampv, rec_thread, grs, gv, it * nx * 4, act);
   }
   if (act) {
     float2 *p0o = reinterpret_cast<float2 *>(p0g + prop * HW);
@@ -727,7 +736,11 @@ __global__ __launch_bounds__(kStripThreads) void adj_strip_kernel(Geom g, int t0
     Q[k - guard] = 0.0f;
     Q[HW + k] = 0.0f;
   }
-  fill_pair_table(tab, A, B, W, pad, 1);
+  // with the c2 * A plane (the native driver, whose forward leaves the rim's history
+  // unspecified) the rim's sponge entries are zero: the imaging sum (A mu) L stays +0
+  // there as with the reference's zero Laplacian, and mu on the rim, which only that
+  // sum, q (zero there through c2 A) and the rim's own state read, is left unspecified
+  fill_pair_table(tab, A, B, W, pad, C2A ? 2 : 1);
   for (int k = t; k < nzx; k += kStripThreads) il[k] = illumg[prop * nzx + k];
 
   float2 m1[R], m2[R], ga[R];
@@ -963,12 +976,12 @@ bool resident_fits(const Geom &g) {
 
 int forward_resident(const Geom &g, int t0, int t1, const float *c2, const float *A, const float *B, const float *w,
                      const float *amp, float *p0, float *p1, float *gathers, float *hist, float *ckpt,
-                     int ckpt_every, hipStream_t st) {
+                     int ckpt_every, hipStream_t st, bool exact_rim) {
   const size_t lds = is_openfwi(g) ? fwd_strip_lds_bytes(g) : fwd_lds_bytes(g);
   return dispatch_fwd(g, [&](auto kc, auto fixed) -> int {
     constexpr bool F = decltype(fixed)::value;
-    // the fixed 150 x 150 plane runs the paired kernel (11 pairs per thread)
-    auto kern = F ? fwd_strip_kernel<70, 70, 40, 1>
+    // the fixed 150 x 150 plane runs the column-strip kernel
+    auto kern = F ? (exact_rim ? fwd_strip_kernel<70, 70, 40, 1, true> : fwd_strip_kernel<70, 70, 40, 1, false>)
                   : fwd_resident_kernel<decltype(kc)::value, F ? 70 : 0, F ? 70 : 0, F ? 40 : 0, F ? 1 : -1>;
     hipError_t e = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return hip_status(e, "forward_resident attribute");

@@ -264,7 +264,9 @@ int fwd_range(Impl impl, const Geom &g, int t0, int t1, const Coef<T> &c, T *p0,
   int rc;
   if constexpr (std::is_same<T, float>::value) {
     if (impl == kResident)
-      rc = forward_resident(g, t0, t1, c.c2, c.A, c.B, c.w, c.amp, p0, p1, gathers, hist, ckpt, every, st);
+      // with the c2 * A adjoint the history's rim is never read
+      rc = forward_resident(g, t0, t1, c.c2, c.A, c.B, c.w, c.amp, p0, p1, gathers, hist, ckpt, every, st,
+                            c.c2a == nullptr);
     else
       rc = forward_stepwise<T>(g, t0, t1, c.c2, c.A, c.B, c.w, c.amp, p0, p1, gathers, hist, ckpt, every, st);
   } else {
