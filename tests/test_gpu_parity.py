@@ -470,3 +470,34 @@ def test_public_resident_adjoint_equals_stepwise():
         got = run(_lib.IMPL_RESIDENT, cuts)
         for name, a, b in zip(('m1', 'm2', 'imaging sum', 'illumination', 'source term'), got, want):
             assert torch.equal(a, b), f'{name}, segments {cuts}'
+
+
+def test_public_resident_forward_equals_stepwise():
+    """The forward entry point (ss_forward_f32) on the 150 x 150 plane with the resident
+    kernels equals the stepwise kernels bit for bit: gathers, the Laplacian history
+    including its zero two-cell rim (gpu.py:35), and the final state, over the whole
+    record and over two segments of odd length (gpu.py:111-136)."""
+    case('public resident forward vs stepwise, whole record and two segments')
+    sim = gpu.Sim(device='cuda')
+    g = sim.geom
+    prep = sim._prepare(np.repeat(synth.mixed(1, seed=12), 2, 0))
+    nb, H, W = prep['nb'], sim.H, sim.W
+    geom = sim._geom(nb)
+
+    def run(impl, cuts):
+        p0 = torch.zeros(nb, g.ns, H, W, device=DEV)
+        p1 = torch.zeros_like(p0)
+        out = torch.empty(nb, g.ns, g.nt, g.nx, device=DEV)
+        hists = []
+        for t0, t1 in cuts:
+            h = torch.empty(nb, g.ns, t1 - t0, H, W, device=DEV)
+            sim._forward(geom, t0, t1, prep['c2'], prep['amp'], p0, p1, gathers=out, hist=h, impl=impl)
+            hists.append(h)
+        return out, torch.cat(hists, 2), p0, p1
+
+    want = run(_lib.IMPL_STEPWISE, [(0, g.nt)])
+    assert float(want[1][..., :2, :].abs().max()) == 0.0 and float(want[1][..., :, -2:].abs().max()) == 0.0
+    for cuts in ([(0, g.nt)], [(0, 377), (377, g.nt)]):
+        got = run(_lib.IMPL_RESIDENT, cuts)
+        for name, a, b in zip(('gathers', 'history', 'p0', 'p1'), got, want):
+            assert torch.equal(a, b), f'{name}, segments {cuts}'
