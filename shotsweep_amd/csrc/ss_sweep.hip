@@ -382,7 +382,12 @@ __global__ __launch_bounds__(kThreads) void sweep_adj_kernel(SArgs a) {
 // same for both cells (their ranges start on even columns).  Two descriptor
 // words per pair: A = plane offset of the first cell | flags (one source flag
 // per cell), B = the two cells' sponge-table byte offsets.  Per cell the
-// arithmetic is that of the cell kernels above.
+// arithmetic is that of the cell kernels above.  Step s (1-based) computes only
+// the rows at least 2s rows inside a cut edge of the extended region (an edge
+// inside the grid): they are the rows still exact after step s, they read only
+// rows computed at step s-1, and they contain the own tile up to s = T.  The
+// rows left out are never read again and never stored, so whole waves skip
+// them while the band shrinks by two rows per step at each cut edge.
 constexpr uint32_t kQInt = 1u << 17, kQOwn = 1u << 18, kQRow = 1u << 19, kQOwnModel = 1u << 20;
 constexpr uint32_t kTabOff = 0x3ff8u;  // table byte offsets of 8-byte entries
 
@@ -463,6 +468,7 @@ __global__ __launch_bounds__(kThreads) void sweep_fwd_pair_kernel(SArgs a) {
   const int rec_e = (r - v.ey0) * EW + (xa + t - v.ex0);
   float *grec = G.gathers ? G.gathers + (int64_t)v.prop * G.nt * G.nx + (xa - pad) + t : nullptr;
   const float ampv = G.amp[v.prop];
+  const int cut_lo = v.ey0 > 0 ? 2 * EW2 : 0, cut_hi = v.ey0 + v.EH < H ? 2 * EW2 : 0;
   __syncthreads();
 
   for (int st = 0; st < n; ++st) {
@@ -471,10 +477,11 @@ __global__ __launch_bounds__(kThreads) void sweep_fwd_pair_kernel(SArgs a) {
     const float wsrc = G.w[it] * ampv;
     const float2 *C2 = reinterpret_cast<const float2 *>(Pc);
     float2 *N2 = reinterpret_cast<float2 *>(Pn);
+    const int plo = cut_lo * (st + 1), phi = NPE - cut_hi * (st + 1);  // shrinking row band
 #pragma unroll
     for (int j = 0; j < KP; ++j) {
       const int p = t + j * kThreads;
-      if (p < NPE) {
+      if (p >= plo && p < phi) {
         const uint32_t d = da[j];
         const float2 r0 = C2[p - 1], r1 = C2[p], r2 = C2[p + 1];
         const float2 um = C2[p - EW2], u2m = C2[p - EW], up = C2[p + EW2], u2p = C2[p + EW];
@@ -600,6 +607,7 @@ __global__ __launch_bounds__(kThreads) void sweep_adj_pair_kernel(SArgs a) {
 #pragma unroll
   for (int j = 0; j < D; ++j)
     Lb[j] = buf_load2(hrs, (da[j] & kQOwn) ? (int)(da[j] & kOffMask) * 4 : kNoAccess, (hi - 1 - G.tlo) * HW * 4);
+  const int cut_lo = v.ey0 > 0 ? 2 * EW2 : 0, cut_hi = v.ey0 + v.EH < H ? 2 * EW2 : 0;
   __syncthreads();
 
   for (int st = 0; st < n; ++st) {
@@ -608,6 +616,7 @@ __global__ __launch_bounds__(kThreads) void sweep_adj_pair_kernel(SArgs a) {
     float2 msrc = make_float2(0.0f, 0.0f);  // the new state of the source's pair (its owner's lane)
     const float2 *C2 = reinterpret_cast<const float2 *>(Qc);
     float2 *N2 = reinterpret_cast<float2 *>(Qn);
+    const int plo = cut_lo * (st + 1), phi = NPE - cut_hi * (st + 1);  // shrinking row band
 #pragma unroll
     for (int j = 0; j < KP; ++j) {
       const int p = t + j * kThreads;
@@ -618,7 +627,7 @@ __global__ __launch_bounds__(kThreads) void sweep_adj_pair_kernel(SArgs a) {
         const int itn = (j + D < KP || st + 1 == n) ? it : it - 1;
         Lb[j % D] = buf_load2(hrs, (da[jn] & kQOwn) ? (int)(da[jn] & kOffMask) * 4 : kNoAccess, (itn - G.tlo) * HW * 4);
       }
-      if (p < NPE) {
+      if (p >= plo && p < phi) {
         const uint32_t d = da[j];
         const int gi = (int)(d & kOffMask);
         const float2 r0 = C2[p - 1], r1 = C2[p], r2 = C2[p + 1];
