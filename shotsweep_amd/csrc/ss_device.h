@@ -72,6 +72,10 @@ __device__ __forceinline__ void lds_st2(uint32_t byte_addr, float2 v) {
   w.y = v.y;
   *reinterpret_cast<ss_lds_v2f *>((uintptr_t)byte_addr) = w;
 }
+typedef __attribute__((address_space(3))) float ss_lds_float;
+__device__ __forceinline__ void lds_st1(uint32_t byte_addr, float v) {
+  *reinterpret_cast<ss_lds_float *>((uintptr_t)byte_addr) = v;
+}
 __device__ __forceinline__ float4 lds_ld4(uint32_t byte_addr) {
   const ss_v4f v = *reinterpret_cast<ss_lds_cv4f *>((uintptr_t)byte_addr);
   return make_float4(v.x, v.y, v.z, v.w);
@@ -101,5 +105,53 @@ __device__ __forceinline__ void lds_barrier() {
 // n * magic / 2^32 exceeds n / d by less than n / 2^32 < 1/d, which is no more
 // than the distance from n / d up to the next integer.
 __device__ __forceinline__ uint32_t magic_div(uint32_t n, uint32_t magic) { return __umulhi(n, magic); }
+
+// Pair descriptor of the paired kernels (32 bits, one per cell pair (2q, 2q+1)):
+//   bit 0       Laplacian interior (both cells share it: the two-cell rim has even width)
+//   bits 4-14   byte offset of the pair's 16-byte sponge entry in the pair table,
+//               which starts at LDS address 0 (a lookup is one mask and one read)
+//   bits 16-30  model pairs: byte offset of the pair's illumination in the model plane
+//   bit 31      model pair (both cells are model cells: the model starts on an even
+//               column and has even width), so one signed compare tests it
+// With pad and the model width even, a pair starts on an even column and the
+// sponge depth of its first cell, dx0, fixes the second's: 0 in the model
+// columns, dx0 - 1 in the left sponge (dx0 even), dx0 + 1 in the right one
+// (dx0 odd).  So the pair table has one entry per (dz, dx0), dz, dx0 in
+// [0, pad], holding both cells' coefficients: (A0, B0, A1, B1) for the forward,
+// (2A0, A0 B0, 2A1, A1 B1) for the adjoint (2A is exact and A = 0.5 * 2A).
+// Model pairs sit at sponge depth 0, entry 0 (A = B = 1).
+constexpr uint32_t kPInterior = 1u, kPTabMask = 0x7ff0u, kPIlMask = 0x7fffu;
+__device__ __forceinline__ uint32_t pair_desc(int q, int H, int W, int nz, int nx, int pad) {
+  const int c = 2 * q, z = c / W, x = c - z * W;
+  uint32_t v = (uint32_t)(sponge_dist(z, H, pad) * (pad + 1) + sponge_dist(x, W, pad)) << 4;
+  if (z >= 2 && z < H - 2 && x >= 2 && x < W - 2) v |= kPInterior;
+  if (z >= pad && z < pad + nz && x >= pad && x < pad + nx)
+    v |= 0x80000000u | ((uint32_t)(((z - pad) * nx + (x - pad)) * 4) << 16);
+  return v;
+}
+
+// Pair table entry (dz, dx0) from the full A, B planes at a representative pair
+// of that depth in the top sponge (row pad - dz): columns pad (dx0 = 0),
+// pad - dx0 (dx0 even, left) or W - 1 - pad + dx0 (dx0 odd, right).
+// mode 0: (A0, B0, A1, B1); mode 1: (2A0, A0 B0, 2A1, A1 B1); mode 2: as mode 1
+// but zero for the pairs on the two-cell rim (dz or dx0 at least pad - 1).
+__device__ inline void fill_pair_table(float4 *tab, const float *A, const float *B, int W, int pad, int mode) {
+  const int n = (pad + 1) * (pad + 1);
+  for (int k = threadIdx.x; k < n; k += blockDim.x) {
+    const int dz = k / (pad + 1), dx0 = k - dz * (pad + 1);
+    const int x0 = dx0 == 0 ? pad : ((dx0 & 1) ? W - 1 - pad + dx0 : pad - dx0);
+    const int c = (pad - dz) * W + x0;
+    const float a0 = A[c], b0 = B[c], a1 = A[c + 1], b1 = B[c + 1];
+    tab[k] = mode ? make_float4(2.0f * a0, a0 * b0, 2.0f * a1, a1 * b1) : make_float4(a0, b0, a1, b1);
+    if (mode == 2 && (dz >= pad - 1 || dx0 >= pad - 1)) tab[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  }
+}
+
+// LDS floats of the stencil overhang (two rows and two cells) on each side of a
+// plane of width W, and of a pair table for sponge width pad.  The table sits at
+// LDS address 0, so a pair's entry is read at the byte offset its descriptor
+// holds, with no base add.
+constexpr int pair_guard(int W) { return 2 * W + 2; }
+constexpr int pair_table_floats(int pad) { return (pad + 1) * (pad + 1) * 4; }
 
 }  // namespace ss

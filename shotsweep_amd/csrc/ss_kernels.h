@@ -78,6 +78,15 @@ bool sweep_tuned(int dir, int H, int W, int pad, int64_t nprop);  // tiling tune
 int sweep_launches(const ss_sweep_group &group, int dir);  // launches of one call (< 0: error)
 bool sweep_fits(const Geom &g);                             // within the tiled kernels' limits
 
+// Band kernels (ss_band.hip): planes too large for one workgroup held by a team of
+// workgroups, one band of rows each, that trade their boundary rows every step
+// (cooperative launch).  band_fits: a band layout exists for the grid and direction;
+// band_run runs a sweep group's whole step range in one launch; band_errors reads the
+// count of exchange waits that ran past spin_limit polls (their bands' halos became NaN).
+bool band_fits(const Geom &g, int dir);
+int band_run(const ss_sweep_group &grp, const Geom &g, int dir, int spin_limit, hipStream_t st);
+int band_errors(int *count);
+
 // c2a: NULL, or the plane c2 * A of c2a_plane (used by the 150 x 150 paired kernel,
 // see resident_uses_c2a; other shapes ignore it).
 int adjoint_resident(const Geom &g, int t0, int t1, const float *c2, const float *A, const float *B, const float *w,

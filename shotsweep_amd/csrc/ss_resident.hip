@@ -87,46 +87,6 @@ __device__ __forceinline__ uint32_t cell_desc(int i, int H, int W, int nz, int n
   return v;
 }
 
-// Pair descriptor of the paired kernels (32 bits, one per cell pair (2q, 2q+1)):
-//   bit 0       Laplacian interior (both cells share it: the two-cell rim has even width)
-//   bits 4-14   byte offset of the pair's 16-byte sponge entry in the pair table,
-//               which starts at LDS address 0 (a lookup is one mask and one read)
-//   bits 16-30  model pairs: byte offset of the pair's illumination in the model plane
-//   bit 31      model pair (both cells are model cells: the model starts on an even
-//               column and has even width), so one signed compare tests it
-// With pad and the model width even, a pair starts on an even column and the
-// sponge depth of its first cell, dx0, fixes the second's: 0 in the model
-// columns, dx0 - 1 in the left sponge (dx0 even), dx0 + 1 in the right one
-// (dx0 odd).  So the pair table has one entry per (dz, dx0), dz, dx0 in
-// [0, pad], holding both cells' coefficients: (A0, B0, A1, B1) for the forward,
-// (2A0, A0 B0, 2A1, A1 B1) for the adjoint (2A is exact and A = 0.5 * 2A).
-// Model pairs sit at sponge depth 0, entry 0 (A = B = 1).
-constexpr uint32_t kPInterior = 1u, kPTabMask = 0x7ff0u, kPIlMask = 0x7fffu;
-__device__ __forceinline__ uint32_t pair_desc(int q, int H, int W, int nz, int nx, int pad) {
-  const int c = 2 * q, z = c / W, x = c - z * W;
-  uint32_t v = (uint32_t)(sponge_dist(z, H, pad) * (pad + 1) + sponge_dist(x, W, pad)) << 4;
-  if (z >= 2 && z < H - 2 && x >= 2 && x < W - 2) v |= kPInterior;
-  if (z >= pad && z < pad + nz && x >= pad && x < pad + nx)
-    v |= 0x80000000u | ((uint32_t)(((z - pad) * nx + (x - pad)) * 4) << 16);
-  return v;
-}
-
-// Pair table entry (dz, dx0) from the full A, B planes at a representative pair
-// of that depth in the top sponge (row pad - dz): columns pad (dx0 = 0),
-// pad - dx0 (dx0 even, left) or W - 1 - pad + dx0 (dx0 odd, right).
-// mode 0: (A0, B0, A1, B1); mode 1: (2A0, A0 B0, 2A1, A1 B1); mode 2: as mode 1
-// but zero for the pairs on the two-cell rim (dz or dx0 at least pad - 1).
-__device__ void fill_pair_table(float4 *tab, const float *A, const float *B, int W, int pad, int mode) {
-  const int n = (pad + 1) * (pad + 1);
-  for (int k = threadIdx.x; k < n; k += blockDim.x) {
-    const int dz = k / (pad + 1), dx0 = k - dz * (pad + 1);
-    const int x0 = dx0 == 0 ? pad : ((dx0 & 1) ? W - 1 - pad + dx0 : pad - dx0);
-    const int c = (pad - dz) * W + x0;
-    const float a0 = A[c], b0 = B[c], a1 = A[c + 1], b1 = B[c + 1];
-    tab[k] = mode ? make_float4(2.0f * a0, a0 * b0, 2.0f * a1, a1 * b1) : make_float4(a0, b0, a1, b1);
-    if (mode == 2 && (dz >= pad - 1 || dx0 >= pad - 1)) tab[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-  }
-}
 
 // Per-cell descriptors are loop-invariant; left visible, the compiler hoists
 // every value derived from them (table addresses, flag masks) out of the time
@@ -188,10 +148,6 @@ size_t adj_lds_bytes(const Geom &g) {
 }
 
 // Strip kernels: [pair table | illumination (adjoint) | guard | plane | guard].
-// The table sits at LDS address 0, so a pair's table entry is read at the byte
-// offset its descriptor holds, with no base add.
-constexpr int pair_guard(int W) { return 2 * W + 2; }
-constexpr int pair_table_floats(int pad) { return (pad + 1) * (pad + 1) * 4; }
 size_t fwd_strip_lds_bytes(const Geom &g) {
   return (size_t)(pair_table_floats(g.pad) + g.H * g.W + 2 * pair_guard(g.W)) * sizeof(float);
 }

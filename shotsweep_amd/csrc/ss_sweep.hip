@@ -777,6 +777,21 @@ std::string tune_key(int dir, int H, int W, int pad, int64_t nprop) {
   return buf;
 }
 
+// Polls of a band exchange before it counts as failed: about a second, against a
+// few microseconds per exchange when every workgroup of a team is resident.
+constexpr int kBandSpin = 1 << 20;
+
+// A group runs on the band kernels when a band layout exists for its grid, the
+// caller asks for no particular tiling, and the call covers the whole record from
+// zero state (the full-history sweeps, which run alone on their stream).
+bool use_band(const ss_sweep_group &in, int dir, Geom *g) {
+  if (check_geom(&in.geom, g) != SS_OK || !band_fits(*g, dir)) return false;
+  // tags of the exchange words count at most 2^20 steps per team (at least 8 teams)
+  const int64_t rounds = ((int64_t)g->nb * g->ns + 7) / 8;
+  return in.tile_h == 0 && in.tile_w == 0 && in.depth == 0 && !in.resume && in.t0 == 0 &&
+         (in.t1 == 0 || in.t1 == g->nt) && rounds * (g->nt + 2) < (1 << 20);
+}
+
 int fill_group(const ss_sweep_group &in, int dir, SGroup *o, char *err, size_t errn) {
   Geom g;
   int rc = check_geom(&in.geom, &g);
@@ -1129,7 +1144,15 @@ struct Floors {
   bool set = false, paired = false;
   int K = 0;
   size_t lds = 0;
+  int ehw = 0, ew = 0, pad = 0;  // largest extended region, widest one and widest sponge so far
 };
+
+// The joint launch sizes its LDS from the largest extended region, the widest one
+// and the widest sponge of all its groups, which may come from different groups.
+bool joint_lds_fits(int H, int W, int pad, int dir, const TileChoice &c, const Floors &fl) {
+  return lds_floats(std::max(ext_cells(H, W, c), fl.ehw), std::max(ext_width(W, c), fl.ew), std::max(pad, fl.pad),
+                    dir, nullptr, nullptr, nullptr) <= kMaxLdsFloats;
+}
 
 // Time the first (up to) six launches of one group alone under a tiling, the
 // faster of two runs after one untimed launch, with the kernel variant and LDS
@@ -1169,8 +1192,24 @@ double time_choice(const ss_sweep_group &grp, int dir, const TileChoice &tc, hip
 }  // namespace
 
 int sweep_run(int ngroups, const ss_sweep_group *groups, int dir, int use_graph, hipStream_t st) {
+  if (ngroups < 1 || ngroups > kMaxGroups || !groups) {
+    set_error("sweep: need 1..4 groups");
+    return SS_EINVAL;
+  }
+  // band groups run whole in one cooperative launch each, the others in the tiled launches
+  std::vector<ss_sweep_group> tiled;
+  for (int i = 0; i < ngroups; ++i) {
+    Geom g;
+    if (use_band(groups[i], dir, &g)) {
+      const int rc = band_run(groups[i], g, dir, kBandSpin, st);
+      if (rc != SS_OK) return rc;
+    } else {
+      tiled.push_back(groups[i]);
+    }
+  }
+  if (tiled.empty()) return SS_OK;
   Plan p;
-  int rc = make_plan(ngroups, groups, dir, &p);
+  int rc = make_plan((int)tiled.size(), tiled.data(), dir, &p);
   if (rc != SS_OK) return rc;
   return run_plan(p, use_graph, st);
 }
@@ -1181,6 +1220,8 @@ bool sweep_tuned(int dir, int H, int W, int pad, int64_t nprop) {
 }
 
 int sweep_launches(const ss_sweep_group &group, int dir) {
+  Geom g;
+  if (use_band(group, dir, &g)) return 1;
   SGroup o;
   char err[256] = {0};
   const int rc = fill_group(group, dir, &o, err, sizeof err);
@@ -1226,13 +1267,16 @@ int sweep_autotune(int ngroups, ss_sweep_group *groups, int dir, int max_trials,
   for (int oi = 0; oi < ngroups && rc == SS_OK; ++oi) {
     const int i = order[oi];
     Geom g;
+    if (use_band(groups[i], dir, &g)) continue;  // no tiling: the band kernels hold the whole plane
     rc = check_geom(&groups[i].geom, &g);
     if (rc != SS_OK) break;
     const auto cand = candidates(g.H, g.W, g.pad, dir);
     double best = -1.0;
     TileChoice pick{0, 0, 0};
-    const int ntry = std::min<int>((int)cand.size(), max_trials > 0 ? max_trials : 12);
-    for (int c = 0; c < ntry; ++c) {
+    const int ntry = max_trials > 0 ? max_trials : 12;
+    for (int c = 0, tried = 0; c < (int)cand.size() && tried < ntry; ++c) {
+      if (!joint_lds_fits(g.H, g.W, g.pad, dir, cand[c], fl)) continue;  // would not fit beside the larger groups
+      ++tried;
       const double tpc = time_choice(groups[i], dir, cand[c], st, e0, e1, fl);
       if (tpc > 0.0 && (best < 0.0 || tpc < best)) {
         best = tpc;
@@ -1258,6 +1302,9 @@ int sweep_autotune(int ngroups, ss_sweep_group *groups, int dir, int max_trials,
         fl.lds = std::max(fl.lds, pp.lds);
       }
     }
+    fl.ehw = std::max(fl.ehw, ext_cells(g.H, g.W, pick));
+    fl.ew = std::max(fl.ew, ext_width(g.W, pick));
+    fl.pad = std::max(fl.pad, g.pad);
     std::lock_guard<std::mutex> lock(g_mutex);
     g_tuned[tune_key(dir, g.H, g.W, g.pad, (int64_t)g.nb * g.ns)] = pick;
   }

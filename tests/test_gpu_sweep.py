@@ -225,3 +225,52 @@ def test_uneven_sizes_match_oracle_at_full_size(size):
     em = rel(mimg.cpu().numpy(), fix[f'{tag}_img'])
     print(f'  image rel L2 vs oracle: Sim {e:.3e}, mixed sweep {em:.3e}')
     assert e < 1e-5 and em < 1e-5
+
+
+def _forward_group(sim, prep, nb, state, hist, gathers, tiles=(0, 0, 0)):
+    g = _lib.SSSweepGroup()
+    g.geom = sim._geom(nb)
+    g.tile_h, g.tile_w, g.depth = tiles
+    g.c2, g.A, g.B, g.w = _lib.ptr(prep['c2']), _lib.ptr(sim.A), _lib.ptr(sim.B), _lib.ptr(sim.w)
+    g.amp, g.state = _lib.ptr(prep['amp']), _lib.ptr(state)
+    g.hist, g.gathers = _lib.ptr(hist), _lib.ptr(gathers)
+    return (_lib.SSSweepGroup * 1)(g)
+
+
+@pytest.mark.parametrize('shape', [(200, 200, 10, 7, (94, 94, 8)), (100, 140, 7, 19, (60, 110, 6))])
+def test_band_forward_bit_identical_to_stepwise(shape):
+    """The band kernels, which hold a plane in a team of workgroups that trade
+    boundary rows every step: 280 x 280 planes (200 x 200 models, pad 40) in four
+    bands, 7 rows x 10 shots = 70 propagations, two rounds of the 64 teams; 180 x
+    220 planes (100 x 140 models) in two bands, 19 rows x 7 shots = 133
+    propagations, two rounds of the 128 teams (the second rounds partial).
+    Gathers, the whole Laplacian history (zero on the rim) and the final state
+    equal the stepwise kernels' bit for bit; the state buffers are NaN on entry,
+    which a zero-state call must not read.  The tiled kernels under a forced
+    tiling of the same grid give the same gathers and history."""
+    nz, nx, ns, nb, tiling = shape
+    acq = Acquisition(nz=nz, nx=nx, nt=300, ns=ns)
+    case(f'band forward {acq.nz}x{acq.nx} nt {acq.nt} ns {acq.ns} x {nb} rows vs stepwise')
+    sim = gpu.Sim(geom=acq)
+    g = acq
+    prep = sim._prepare(models(acq, nb, 11))
+    H, W = sim.H, sim.W
+    p0 = torch.zeros(nb, g.ns, H, W, device=DEV)
+    p1 = torch.zeros_like(p0)
+    ref_out = torch.empty(nb, g.ns, g.nt, g.nx, device=DEV)
+    ref_hist = torch.empty(nb, g.ns, g.nt, H, W, device=DEV)
+    sim._forward(sim._geom(nb), 0, g.nt, prep['c2'], prep['amp'], p0, p1, gathers=ref_out, hist=ref_hist,
+                 impl=_lib.IMPL_STEPWISE)
+    lib = _lib.load()
+    for tiles in [(0, 0, 0), tiling]:
+        state = torch.full((4, nb, g.ns, H, W), float('nan'), device=DEV)
+        hist = torch.full_like(ref_hist, float('nan'))
+        out = torch.full_like(ref_out, float('nan'))
+        arr = _forward_group(sim, prep, nb, state, hist, out, tiles)
+        _lib.check(lib.ss_sweep_forward_f32(1, arr, 1, _lib.stream_handle(DEV)), 'sweep forward')
+        torch.cuda.synchronize()
+        assert torch.equal(out, ref_out), f'gathers, tiles {tiles}'
+        assert torch.equal(hist, ref_hist), f'history, tiles {tiles}'
+        if tiles == (0, 0, 0):  # one launch from state buffer 0: the final state in buffer 1
+            assert torch.equal(state[2], p0) and torch.equal(state[3], p1)
+        print(f'  tiles {tiles}: equal', flush=True)
