@@ -274,3 +274,52 @@ def test_band_forward_bit_identical_to_stepwise(shape):
         if tiles == (0, 0, 0):  # one launch from state buffer 0: the final state in buffer 1
             assert torch.equal(state[2], p0) and torch.equal(state[3], p1)
         print(f'  tiles {tiles}: equal', flush=True)
+
+
+def _adjoint_group(sim, prep, nb, state, res, hist, gacc, illum, sg, tiles=(0, 0, 0)):
+    g = _lib.SSSweepGroup()
+    g.geom = sim._geom(nb)
+    g.tile_h, g.tile_w, g.depth = tiles
+    g.c2, g.A, g.B, g.w = _lib.ptr(prep['c2']), _lib.ptr(sim.A), _lib.ptr(sim.B), _lib.ptr(sim.w)
+    g.amp, g.res, g.state, g.hist = _lib.ptr(prep['sd']), _lib.ptr(res), _lib.ptr(state), _lib.ptr(hist)
+    g.gacc, g.illum, g.src_grad = _lib.ptr(gacc), _lib.ptr(illum), _lib.ptr(sg)
+    return (_lib.SSSweepGroup * 1)(g)
+
+
+@pytest.mark.parametrize('shape', [(200, 200, 10, 7, (70, 94, 6)), (100, 140, 7, 19, (90, 74, 6))])
+def test_band_adjoint_bit_identical_to_stepwise(shape):
+    """The adjoint on the band kernels (the bands trade the new state of their edge
+    rows every step and form the halo's q from it) at the same two shapes and
+    propagation counts as the forward test, with the stepwise forward's Laplacian
+    history and its gathers as the residual: imaging sum, illumination, source term
+    and the final state (m1, m2) equal the stepwise adjoint's bit for bit; the state
+    buffers and outputs are NaN on entry.  The tiled kernels under a forced tiling
+    give the same imaging sum, illumination and source term."""
+    nz, nx, ns, nb, tiling = shape
+    acq = Acquisition(nz=nz, nx=nx, nt=300, ns=ns)
+    case(f'band adjoint {acq.nz}x{acq.nx} nt {acq.nt} ns {acq.ns} x {nb} rows vs stepwise')
+    sim = gpu.Sim(geom=acq)
+    g = acq
+    prep = sim._prepare(models(acq, nb, 12))
+    H, W = sim.H, sim.W
+    res, hist = stepwise_forward(sim, prep)
+    z = lambda *s: torch.zeros(*s, device=DEV)
+    m1, m2 = z(nb, g.ns, H, W), z(nb, g.ns, H, W)
+    ref_gacc, ref_illum, ref_sg = z(nb, g.ns, H, W), z(nb, g.ns, g.nz, g.nx), z(nb, g.ns)
+    sim._adjoint(sim._geom(nb), 0, g.nt, prep['c2'], prep['sd'], res, hist, m1, m2, ref_gacc, ref_illum, ref_sg,
+                 impl=_lib.IMPL_STEPWISE)
+    assert float(ref_illum.abs().max()) > 0 and float(ref_sg.abs().max()) > 0
+    lib = _lib.load()
+    nan = float('nan')
+    for tiles in [(0, 0, 0), tiling]:
+        state = torch.full((4, nb, g.ns, H, W), nan, device=DEV)
+        gacc, illum, sg = torch.full_like(ref_gacc, nan), torch.full_like(ref_illum, nan), torch.full_like(ref_sg, nan)
+        arr = _adjoint_group(sim, prep, nb, state, res, hist, gacc, illum, sg, tiles)
+        _lib.check(lib.ss_sweep_adjoint_f32(1, arr, 1, _lib.stream_handle(DEV)), 'sweep adjoint')
+        torch.cuda.synchronize()
+        assert torch.equal(gacc, ref_gacc), f'imaging sum, tiles {tiles}'
+        assert torch.equal(illum, ref_illum), f'illumination, tiles {tiles}'
+        assert torch.equal(sg, ref_sg), f'source term, tiles {tiles}'
+        if tiles == (0, 0, 0):  # one launch from state buffer 0: the final state in buffer 1
+            assert torch.equal(state[2], m1) and torch.equal(state[3], m2)
+        print(f'  tiles {tiles}: equal', flush=True)
