@@ -400,6 +400,20 @@ int ss_sweep_adjoint_f32(int ngroups, const ss_sweep_group *groups, int use_grap
  */
 int ss_sweep_autotune_f32(int ngroups, ss_sweep_group *groups, int direction, int max_trials, void *stream);
 
+/*
+ * Propagations per round of the band kernels on the current device, or 0 when
+ * the grid has no band layout.  A group on a grid with a band layout (200 x 200
+ * and 100 x 140 models, pad 40, receivers and sources on row 1) whose tiling is
+ * left at 0 and whose call covers the whole record from zero state runs on the
+ * band kernels: a team of workgroups, one per band of rows, holds each plane
+ * for the whole record and trades the bands' edge rows every step, and one
+ * cooperative launch runs the group's propagations in rounds of this many, so
+ * batches of a multiple of it fill every round.  No counterpart in the
+ * reference (one 70 x 70 geometry, gpu.py:85); it serves batch planning for
+ * ss_sweep_forward_f32 / ss_sweep_adjoint_f32.
+ */
+int ss_sweep_round_f32(const ss_geom *geom);
+
 #ifdef __cplusplus
 }
 #endif

@@ -289,7 +289,7 @@ extern "C" {
 
 const char *ss_last_error(void) { return ss::g_last_error.c_str(); }
 
-int ss_version(void) { return 6; }
+int ss_version(void) { return 7; }
 
 int64_t ss_resident_max_cells(void) { return resident_max_cells(); }
 
@@ -524,6 +524,18 @@ int ss_sweep_autotune_f32(int ngroups, ss_sweep_group *groups, int direction, in
     return sweep_autotune(ngroups, groups, direction, max_trials, (hipStream_t)stream);
   } catch (const std::exception &e) {
     set_error(std::string("sweep autotune: ") + e.what());
+    return SS_EHIP;
+  }
+}
+
+int ss_sweep_round_f32(const ss_geom *geom) {
+  Geom g;
+  const int rc = check_geom(geom, &g);
+  if (rc != SS_OK) return rc;
+  try {
+    return band_round(g);
+  } catch (const std::exception &e) {
+    set_error(std::string("sweep round: ") + e.what());
     return SS_EHIP;
   }
 }

@@ -790,23 +790,36 @@ int band_buffers(int dev, hipStream_t st, int64_t words, int64_t c2a_floats, Ban
   return SS_OK;
 }
 
+// Workgroups of a band launch: one per CU, a multiple of 8 XCDs x NB bands (every
+// team on one XCD), all resident at once; the kernel's LDS attribute is set.
+template <class S>
+int band_grid(int dir, int *dev, int *grid) {
+  constexpr int NB = S::H / S::BH;
+  auto kern = dir ? adj_band_kernel<S> : fwd_band_kernel<S>;
+  const size_t lds = dir ? adj_band_lds_bytes<S>() : band_lds_bytes<S>();
+  int ncu = 0, per_cu = 0;
+  hipError_t e = hipGetDevice(dev);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, *dev);
+  if (e == hipSuccess) e = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kBandThreads, lds);
+  if (e != hipSuccess) return hip_status(e, "band launch setup");
+  *grid = (ncu * std::min(per_cu, 1)) / (kXcds * NB) * (kXcds * NB);
+  if (*grid < kXcds * NB) {
+    set_error("band kernels: the device cannot hold a team of workgroups");
+    return SS_EUNSUP;
+  }
+  return SS_OK;
+}
+
 template <class S>
 int band_launch(const ss_sweep_group &grp, const Geom &g, int dir, int spin_limit, hipStream_t st) {
   constexpr int NB = S::H / S::BH;
   auto kern = dir ? adj_band_kernel<S> : fwd_band_kernel<S>;
   const size_t lds = dir ? adj_band_lds_bytes<S>() : band_lds_bytes<S>();
-  int dev = 0, ncu = 0, per_cu = 0;
-  hipError_t e = hipGetDevice(&dev);
-  if (e == hipSuccess) e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-  if (e == hipSuccess) e = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kBandThreads, lds);
-  if (e != hipSuccess) return hip_status(e, "band launch setup");
-  // every team on one XCD: a multiple of 8 XCDs x NB bands, all resident at once
-  const int grid = (ncu * std::min(per_cu, 1)) / (kXcds * NB) * (kXcds * NB);
-  if (grid < kXcds * NB) {
-    set_error("band kernels: the device cannot hold a team of workgroups");
-    return SS_EUNSUP;
-  }
+  int dev = 0, grid = 0;
+  int rc = band_grid<S>(dir, &dev, &grid);
+  if (rc != SS_OK) return rc;
+  hipError_t e = hipSuccess;
   const int nteams = grid / NB;
   const int64_t nprop = (int64_t)g.nb * g.ns;
   const int nslots = (grp.t1 > 0 ? grp.t1 : g.nt) - grp.t0;
@@ -815,7 +828,7 @@ int band_launch(const ss_sweep_group &grp, const Geom &g, int dir, int spin_limi
     return SS_EUNSUP;
   }
   BandBuffers bb;
-  int rc = band_buffers(dev, st, (int64_t)nteams * S::XTEAM, dir ? (int64_t)g.nb * S::HW : 0, &bb);
+  rc = band_buffers(dev, st, (int64_t)nteams * S::XTEAM, dir ? (int64_t)g.nb * S::HW : 0, &bb);
   if (rc != SS_OK) return rc;
   if (dir) {
     rc = c2a_plane(g.nb, S::H, S::W, static_cast<const float *>(grp.c2), static_cast<const float *>(grp.A), bb.c2a,
@@ -865,6 +878,20 @@ int band_run(const ss_sweep_group &grp, const Geom &g, int dir, int spin_limit, 
   if (band_is<Band180>(g)) return band_launch<Band180>(grp, g, dir, spin_limit, st);
   set_error("band kernels: no band layout for this grid");
   return SS_EUNSUP;
+}
+
+template <class S>
+int band_teams(const Geom &) {
+  int dev = 0, g0 = 0, g1 = 0;
+  int rc = band_grid<S>(0, &dev, &g0);
+  if (rc == SS_OK) rc = band_grid<S>(1, &dev, &g1);
+  return rc != SS_OK ? rc : std::min(g0, g1) / (S::H / S::BH);
+}
+
+int band_round(const Geom &g) {
+  if (band_is<Band280>(g)) return band_teams<Band280>(g);
+  if (band_is<Band180>(g)) return band_teams<Band180>(g);
+  return 0;
 }
 
 int band_errors(int *count) {

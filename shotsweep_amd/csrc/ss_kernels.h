@@ -86,6 +86,7 @@ bool sweep_fits(const Geom &g);                             // within the tiled 
 bool band_fits(const Geom &g, int dir);
 int band_run(const ss_sweep_group &grp, const Geom &g, int dir, int spin_limit, hipStream_t st);
 int band_errors(int *count);
+int band_round(const Geom &g);  // propagations per round of a band launch (0: no band layout)
 
 // c2a: NULL, or the plane c2 * A of c2a_plane (used by the 150 x 150 paired kernel,
 // see resident_uses_c2a; other shapes ignore it).

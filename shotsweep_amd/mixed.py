@@ -19,6 +19,7 @@ the autotuner (``ss_sweep_autotune_f32``) on the first call.  When the
 Laplacian histories of all groups exceed the memory budget, the models are
 split into chunks, each a proportional slice of every group.
 """
+import ctypes
 import math
 
 import numpy as np
@@ -48,6 +49,7 @@ class MixedSweep:
         self.tiles = [dict(tiles[k]) if tiles is not None else {} for k in range(len(acquisitions))]
         self._buf = [{} for _ in acquisitions]
         self._plans = {}
+        self._rounds = {}
         self.events = None   # list -> (name, start, end) HIP events around the two sweep launches
 
     # ------------------------------------------------------------- buffers
@@ -191,8 +193,40 @@ class MixedSweep:
         """Device bytes of the buffers this sweep keeps between calls."""
         return sum(t.numel() * t.element_size() for b in self._buf for t in b.values())
 
+    def _round(self, k):
+        """Propagations per round of group k's band launches on this device (0: the
+        group runs on the tiled kernels, where batch size has no such granularity)."""
+        if k not in self._rounds:
+            n = _lib.load().ss_sweep_round_f32(ctypes.byref(self.sims[k]._geom(1)))
+            self._rounds[k] = max(int(n), 0)
+        return self._rounds[k]
+
+    @staticmethod
+    def _split(models, props, per_round, n, cap):
+        """n model counts in [0, cap] summing to ``models``: the fewest band rounds
+        (ceil(props * m / per_round) per chunk), then the most even; largest first."""
+        if models == 0:
+            return [0] * n
+        if per_round <= 0:
+            out = [min(cap, models - c * cap) for c in range(n)]
+            return [max(m, 0) for m in out]
+        best = {0: (0, 0, ())}
+        for _ in range(n):
+            nxt = {}
+            for used, (r, sq, cs) in best.items():
+                for m in range(0, min(cap, models - used) + 1):
+                    cand = (r + -(-props * m // per_round), sq + m * m, cs + (m,))
+                    if used + m not in nxt or cand[:2] < nxt[used + m][:2]:
+                        nxt[used + m] = cand
+            best = nxt
+        return sorted(best[models][2], reverse=True)
+
     def _chunks(self, rows, repeats):
-        """Split each group's rows into the same number of chunks (whole obs rows each).
+        """Split each group's rows into chunks of whole observation rows whose
+        Laplacian histories fit the memory budget.  Each group's buffers are sized
+        by its largest chunk, so the budget bounds the sum over groups of those; a
+        group on the band kernels gets chunk sizes that fill its rounds (a round
+        holds ss_sweep_round_f32 propagations), compared over a few chunk counts.
 
         The plan is fixed per batch shape on first use, so later calls reuse the
         same buffers (and captured graphs)."""
@@ -203,18 +237,32 @@ class MixedSweep:
         if budget is None:
             free, _ = torch.cuda.mem_get_info(self.device)
             budget = 0.6 * (free + self.held_bytes())
-        n = max(1, math.ceil(self.history_bytes(rows) / budget))
-        while True:
-            spans = []
-            for nb, rep in zip(rows, repeats):
-                m = nb // rep
-                per = math.ceil(m / n) if m else 0
-                spans.append([(c * per * rep, min(nb, (c + 1) * per * rep)) for c in range(n)])
-            worst = max(self.history_bytes([s[c][1] - s[c][0] for s in spans]) for c in range(n))
-            if worst <= budget or n >= max(r // p for r, p in zip(rows, repeats) if r):
-                self._plans[key] = (n, spans)
-                return n, spans
-            n += 1
+        groups = []
+        for k, (nb, rep) in enumerate(zip(rows, repeats)):
+            g = self.sims[k].geom
+            per_model = rep * g.ns * g.nt * self.sims[k].H * self.sims[k].W * 4
+            groups.append((nb // rep if nb else 0, rep, rep * g.ns, self._round(k) if nb else 0, per_model, g.nt))
+        most = max([m for m, *_ in groups] + [1])
+        n0 = max(1, math.ceil(self.history_bytes(rows) / budget))
+        best = None
+        for n in range(n0, most + 1):
+            counts = [self._split(m, p, t, n, -(-m // n)) for m, _, p, t, _, _ in groups]
+            held = sum(max(c) * h for c, (_, _, _, _, h, _) in zip(counts, groups))
+            if held > budget and n < most:
+                continue
+            rounds = sum(nt * sum(-(-p * m // t) for m in c)
+                         for c, (_, _, p, t, _, nt) in zip(counts, groups) if t > 0)
+            if best is None or rounds < best[0]:
+                best = (rounds, n, counts)
+            if n >= n0 + 3 or rounds == 0:
+                break
+        _, n, counts = best
+        spans = []
+        for c, (_, rep, *_r) in zip(counts, groups):
+            ends = np.cumsum([0] + list(c)) * rep
+            spans.append([(int(ends[i]), int(ends[i + 1])) for i in range(n)])
+        self._plans[key] = (n, spans)
+        return n, spans
 
     @torch.no_grad()
     def images_device(self, batches, t_mute=0.12, eps=1e-4, laplacian=True):
