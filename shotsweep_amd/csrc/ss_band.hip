@@ -113,45 +113,62 @@ __device__ __forceinline__ XR x_rsrc(uint64_t *xbuf, int team) { return xbuf + (
 
 // The exchange wave: the neighbours' rows of p(t) (tag) into the halo rows, LDS
 // rows 0, 1 (from the band above) and BH + 2, BH + 3 (from the band below).  Every
-// lane loads its words, keeps those whose tag matches and reloads the rest; past
-// spin_limit rounds the missing words become NaN and the error counter rises.
-template <class S>
-__device__ void x_receive(XR xr, int band, int tag, uint32_t band_addr, int lane, int spin_limit,
-                          int *err, bool &failed) {
+// lane loads its words of both sides at once, keeps those whose tag matches and
+// reloads the rest; past spin_limit rounds the missing words become NaN and the
+// error counter rises.  `conv` turns a received word into the halo's value (side,
+// word index, value).
+template <class S, class Conv>
+__device__ void x_receive_sides(XR xr, int band, int tag, uint32_t halo_addr, int lane, int spin_limit, int *err,
+                                bool &failed, Conv conv) {
   constexpr int NB = S::H / S::BH, NW = S::XSIDE, NWL = (NW + 63) / 64;
   constexpr uint32_t kRow = 4u * S::W;
+  const bool has[2] = {band > 0, band < NB - 1};
+  const int src[2] = {x_side<S>(band - 1, 1, tag), x_side<S>(band + 1, 0, tag)};
+  const uint32_t dst[2] = {halo_addr, halo_addr + (uint32_t)(S::BH + 2) * kRow};
+  uint64_t pending = 0;  // bit sd * NWL + k: word lane + 64 k of side sd still missing
 #pragma unroll
-  for (int sd = 0; sd < 2; ++sd) {
-    if (sd == 0 ? band == 0 : band == NB - 1) continue;
-    const int src = x_side<S>(sd == 0 ? band - 1 : band + 1, sd == 0 ? 1 : 0, tag);
-    const uint32_t dst = band_addr + (sd == 0 ? 0u : (uint32_t)(S::BH + 2) * kRow);
-    uint32_t pending = 0;
+  for (int sd = 0; sd < 2; ++sd)
 #pragma unroll
     for (int k = 0; k < NWL; ++k)
-      if (lane + 64 * k < NW) pending |= 1u << k;
-    for (int spins = 0;; ++spins) {
-      uint64_t v[NWL];
+      if (has[sd] && lane + 64 * k < NW) pending |= 1ull << (sd * NWL + k);
+  for (int spins = 0; pending != 0 || __builtin_amdgcn_ballot_w64(pending != 0); ++spins) {
+    uint64_t v[2 * NWL];
 #pragma unroll
-      for (int k = 0; k < NWL; ++k) v[k] = (pending >> k) & 1u ? x_get(xr, (lane + 64 * k) * 8, src) : 0;
+    for (int sd = 0; sd < 2; ++sd)
+#pragma unroll
+      for (int k = 0; k < NWL; ++k)
+        v[sd * NWL + k] = (pending >> (sd * NWL + k)) & 1u ? x_get(xr, (lane + 64 * k) * 8, src[sd]) : 0;
+#pragma unroll
+    for (int sd = 0; sd < 2; ++sd)
 #pragma unroll
       for (int k = 0; k < NWL; ++k) {
-        if (((pending >> k) & 1u) && (int)(uint32_t)(v[k] >> 32) == tag) {
-          lds_st1(dst + 4u * (uint32_t)(lane + 64 * k), __builtin_bit_cast(float, (uint32_t)v[k]));
-          pending &= ~(1u << k);
+        const int b = sd * NWL + k;
+        if (((pending >> b) & 1u) && (int)(uint32_t)(v[b] >> 32) == tag) {
+          const int w = lane + 64 * k;
+          lds_st1(dst[sd] + 4u * (uint32_t)w, conv(sd, w, __builtin_bit_cast(float, (uint32_t)v[b])));
+          pending &= ~(1ull << b);
         }
       }
-      if (!__builtin_amdgcn_ballot_w64(pending != 0)) break;
-      if (failed || spins >= spin_limit) {
-        if (!failed && lane == 0) atomicAdd(err, 1);
-        failed = true;
+    if (!__builtin_amdgcn_ballot_w64(pending != 0)) break;
+    if (failed || spins >= spin_limit) {
+      if (!failed && lane == 0) atomicAdd(err, 1);
+      failed = true;
+#pragma unroll
+      for (int sd = 0; sd < 2; ++sd)
 #pragma unroll
         for (int k = 0; k < NWL; ++k)
-          if ((pending >> k) & 1u) lds_st1(dst + 4u * (uint32_t)(lane + 64 * k), __builtin_nanf(""));
-        break;
-      }
-      __builtin_amdgcn_s_sleep(1);
+          if ((pending >> (sd * NWL + k)) & 1u) lds_st1(dst[sd] + 4u * (uint32_t)(lane + 64 * k), __builtin_nanf(""));
+      break;
     }
+    __builtin_amdgcn_s_sleep(1);
   }
+}
+
+template <class S>
+__device__ void x_receive(XR xr, int band, int tag, uint32_t band_addr, int lane, int spin_limit, int *err,
+                          bool &failed) {
+  x_receive_sides<S>(xr, band, tag, band_addr, lane, spin_limit, err, failed,
+                     [](int, int, float v) { return v; });
 }
 
 // One row k of a strip: p(t+1) into PO[k] from p(t) (PC, the vertical taps zm2 ..
@@ -422,46 +439,14 @@ constexpr size_t adj_band_lds_bytes() {
 // halo's q = ((c2 A) m) / 12, c2 A from the coefficient rows (zero on the rim,
 // where q is +0).
 template <class S>
-__device__ void x_receive_q(XR xr, int band, int tag, uint32_t q_addr, uint32_t coef_addr,
-                            int lane, int spin_limit, int *err, bool &failed) {
-  constexpr int NB = S::H / S::BH, NW = S::XSIDE, NWL = (NW + 63) / 64;
+__device__ void x_receive_q(XR xr, int band, int tag, uint32_t q_addr, uint32_t coef_addr, int lane, int spin_limit,
+                            int *err, bool &failed) {
   constexpr uint32_t kRow = 4u * S::W;
-#pragma unroll
-  for (int sd = 0; sd < 2; ++sd) {
-    if (sd == 0 ? band == 0 : band == NB - 1) continue;
-    const int src = x_side<S>(sd == 0 ? band - 1 : band + 1, sd == 0 ? 1 : 0, tag);
-    const uint32_t dst = q_addr + (sd == 0 ? 0u : (uint32_t)(S::BH + 2) * kRow);
-    const uint32_t cf = coef_addr + (uint32_t)sd * 2u * kRow;
-    uint32_t pending = 0;
-#pragma unroll
-    for (int k = 0; k < NWL; ++k)
-      if (lane + 64 * k < NW) pending |= 1u << k;
-    for (int spins = 0;; ++spins) {
-      uint64_t v[NWL];
-#pragma unroll
-      for (int k = 0; k < NWL; ++k) v[k] = (pending >> k) & 1u ? x_get(xr, (lane + 64 * k) * 8, src) : 0;
-#pragma unroll
-      for (int k = 0; k < NWL; ++k) {
-        if (((pending >> k) & 1u) && (int)(uint32_t)(v[k] >> 32) == tag) {
-          const uint32_t off = 4u * (uint32_t)(lane + 64 * k);
-          const float ca = lds_table_a(cf + off);
-          const float q = div12(ca * __builtin_bit_cast(float, (uint32_t)v[k]));
-          lds_st1(dst + off, ca != 0.0f ? q : 0.0f);
-          pending &= ~(1u << k);
-        }
-      }
-      if (!__builtin_amdgcn_ballot_w64(pending != 0)) break;
-      if (failed || spins >= spin_limit) {
-        if (!failed && lane == 0) atomicAdd(err, 1);
-        failed = true;
-#pragma unroll
-        for (int k = 0; k < NWL; ++k)
-          if ((pending >> k) & 1u) lds_st1(dst + 4u * (uint32_t)(lane + 64 * k), __builtin_nanf(""));
-        break;
-      }
-      __builtin_amdgcn_s_sleep(1);
-    }
-  }
+  x_receive_sides<S>(xr, band, tag, q_addr, lane, spin_limit, err, failed, [&](int sd, int w, float m) {
+    const float ca = lds_table_a(coef_addr + (uint32_t)sd * 2u * kRow + 4u * (uint32_t)w);
+    const float q = div12(ca * m);
+    return ca != 0.0f ? q : 0.0f;
+  });
 }
 
 template <class S>
