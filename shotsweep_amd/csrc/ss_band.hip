@@ -876,6 +876,12 @@ int band_teams(const Geom &) {
 int band_round(const Geom &g) {
   if (band_is<Band280>(g)) return band_teams<Band280>(g);
   if (band_is<Band180>(g)) return band_teams<Band180>(g);
+  if (resident_uses_c2a(g)) {  // the column-strip kernels: one workgroup per CU
+    int dev = 0, ncu = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    return e == hipSuccess ? ncu : hip_status(e, "sweep round");
+  }
   return 0;
 }
 

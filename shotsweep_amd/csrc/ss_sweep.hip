@@ -781,6 +781,54 @@ std::string tune_key(int dir, int H, int W, int pad, int64_t nprop) {
 // few microseconds per exchange when every workgroup of a team is resident.
 constexpr int kBandSpin = 1 << 20;
 
+// A whole-record call from zero state with no tiling asked for (the full-history
+// sweeps, which run alone on their stream).
+bool whole_record(const ss_sweep_group &in, const Geom &g) {
+  return in.tile_h == 0 && in.tile_w == 0 && in.depth == 0 && !in.resume && in.t0 == 0 &&
+         (in.t1 == 0 || in.t1 == g.nt);
+}
+
+// Such a call on the 150 x 150 plane (70 x 70 models, pad 40, row 1) runs on the
+// column-strip kernels, one workgroup per propagation for the whole record; the
+// state stays in state buffer `phase` (no launch of the tiled sequence).
+bool use_resident(const ss_sweep_group &in, Geom *g) {
+  return check_geom(&in.geom, g) == SS_OK && resident_uses_c2a(*g) && whole_record(in, *g);
+}
+
+__global__ void zero_kernel(float *p, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = 0.0f;
+}
+
+// zero fill as a kernel (not a memset node, should a caller capture the call)
+int zero_fill(float *p, int64_t n, hipStream_t st) {
+  if (n <= 0) return SS_OK;
+  const unsigned blocks = (unsigned)std::min<int64_t>((n + 255) / 256, 4096);
+  zero_kernel<<<dim3(blocks), dim3(256), 0, st>>>(p, n);
+  return hip_status(hipGetLastError(), "sweep zero fill");
+}
+
+int resident_run(const ss_sweep_group &in, const Geom &g, int dir, hipStream_t st) {
+  const int64_t nprop = (int64_t)g.nb * g.ns, PS = nprop * g.H * g.W;
+  float *s0 = static_cast<float *>(in.state) + 2 * (in.phase & 1) * PS, *s1 = s0 + PS;
+  const float *c2 = static_cast<const float *>(in.c2), *A = static_cast<const float *>(in.A),
+              *B = static_cast<const float *>(in.B), *w = static_cast<const float *>(in.w),
+              *amp = static_cast<const float *>(in.amp);
+  int rc = zero_fill(s0, 2 * PS, st);
+  if (rc != SS_OK) return rc;
+  if (dir == 0)
+    return forward_resident(g, 0, g.nt, c2, A, B, w, amp, s0, s1, static_cast<float *>(in.gathers),
+                            static_cast<float *>(in.hist), nullptr, 1, st, true);
+  float *gacc = static_cast<float *>(in.gacc), *illum = static_cast<float *>(in.illum),
+        *sg = static_cast<float *>(in.src_grad);
+  if (rc == SS_OK) rc = zero_fill(gacc, PS, st);
+  if (rc == SS_OK) rc = zero_fill(illum, nprop * g.nz * g.nx, st);
+  if (rc == SS_OK) rc = zero_fill(sg, nprop, st);
+  if (rc != SS_OK) return rc;
+  return adjoint_resident(g, 0, g.nt, c2, A, B, w, amp, static_cast<const float *>(in.res),
+                          static_cast<const float *>(in.hist), s0, s1, gacc, illum, sg, st, nullptr);
+}
+
 // A group runs on the band kernels when a band layout exists for its grid, the
 // caller asks for no particular tiling, and the call covers the whole record from
 // zero state (the full-history sweeps, which run alone on their stream).
@@ -1203,6 +1251,9 @@ int sweep_run(int ngroups, const ss_sweep_group *groups, int dir, int use_graph,
     if (use_band(groups[i], dir, &g)) {
       const int rc = band_run(groups[i], g, dir, kBandSpin, st);
       if (rc != SS_OK) return rc;
+    } else if (use_resident(groups[i], &g)) {
+      const int rc = resident_run(groups[i], g, dir, st);
+      if (rc != SS_OK) return rc;
     } else {
       tiled.push_back(groups[i]);
     }
@@ -1222,6 +1273,7 @@ bool sweep_tuned(int dir, int H, int W, int pad, int64_t nprop) {
 int sweep_launches(const ss_sweep_group &group, int dir) {
   Geom g;
   if (use_band(group, dir, &g)) return 1;
+  if (use_resident(group, &g)) return 0;  // state read and written in place: buffer `phase`
   SGroup o;
   char err[256] = {0};
   const int rc = fill_group(group, dir, &o, err, sizeof err);
@@ -1267,7 +1319,7 @@ int sweep_autotune(int ngroups, ss_sweep_group *groups, int dir, int max_trials,
   for (int oi = 0; oi < ngroups && rc == SS_OK; ++oi) {
     const int i = order[oi];
     Geom g;
-    if (use_band(groups[i], dir, &g)) continue;  // no tiling: the band kernels hold the whole plane
+    if (use_band(groups[i], dir, &g) || use_resident(groups[i], &g)) continue;  // no tiling: whole planes held
     rc = check_geom(&groups[i].geom, &g);
     if (rc != SS_OK) break;
     const auto cand = candidates(g.H, g.W, g.pad, dir);

@@ -20,6 +20,7 @@ Laplacian histories of all groups exceed the memory budget, the models are
 split into chunks, each a proportional slice of every group.
 """
 import ctypes
+import itertools
 import math
 
 import numpy as np
@@ -224,9 +225,12 @@ class MixedSweep:
     def _chunks(self, rows, repeats):
         """Split each group's rows into chunks of whole observation rows whose
         Laplacian histories fit the memory budget.  Each group's buffers are sized
-        by its largest chunk, so the budget bounds the sum over groups of those; a
-        group on the band kernels gets chunk sizes that fill its rounds (a round
-        holds ss_sweep_round_f32 propagations), compared over a few chunk counts.
+        by its largest chunk, so the budget bounds the sum over groups of those.  A
+        group whose propagations run in rounds (the band kernels, or the column-strip
+        kernels' one workgroup per CU: ss_sweep_round_f32) gets chunk sizes that fill
+        its rounds, and may take larger chunks than an even split where that saves
+        rounds; the plan minimises the rounds (weighted by step count) over a few
+        chunk counts.
 
         The plan is fixed per batch shape on first use, so later calls reuse the
         same buffers (and captured graphs)."""
@@ -237,26 +241,40 @@ class MixedSweep:
         if budget is None:
             free, _ = torch.cuda.mem_get_info(self.device)
             budget = 0.6 * (free + self.held_bytes())
-        groups = []
+        groups = []  # (models, rows per model, propagations per model, per round, bytes per model, steps)
         for k, (nb, rep) in enumerate(zip(rows, repeats)):
             g = self.sims[k].geom
             per_model = rep * g.ns * g.nt * self.sims[k].H * self.sims[k].W * 4
             groups.append((nb // rep if nb else 0, rep, rep * g.ns, self._round(k) if nb else 0, per_model, g.nt))
         most = max([m for m, *_ in groups] + [1])
         n0 = max(1, math.ceil(self.history_bytes(rows) / budget))
-        best = None
+        best, smallest = None, None
         for n in range(n0, most + 1):
-            counts = [self._split(m, p, t, n, -(-m // n)) for m, _, p, t, _, _ in groups]
-            held = sum(max(c) * h for c, (_, _, _, _, h, _) in zip(counts, groups))
-            if held > budget and n < most:
-                continue
-            rounds = sum(nt * sum(-(-p * m // t) for m in c)
-                         for c, (_, _, p, t, _, nt) in zip(counts, groups) if t > 0)
-            if best is None or rounds < best[0]:
-                best = (rounds, n, counts)
-            if n >= n0 + 3 or rounds == 0:
+            options = []  # per group: (weighted rounds, bytes held, counts), fewer rounds per larger cap
+            for m, _, p, t, h, nt in groups:
+                even = -(-m // n) if m else 0
+                if t <= 0 or m == 0:
+                    c = self._split(m, p, t, n, even)
+                    options.append([(0, max(c) * h, c)])
+                    continue
+                opts, last = [], None
+                for cap in range(even, m + 1):
+                    c = self._split(m, p, t, n, cap)
+                    r = sum(-(-p * x // t) for x in c)
+                    if last is None or r < last:
+                        opts.append((r * nt, max(c) * h, c))
+                        last = r
+                options.append(opts)
+            for combo in itertools.product(*options):
+                held = sum(o[1] for o in combo)
+                plan = (sum(o[0] for o in combo), n, [o[2] for o in combo])
+                if smallest is None or held < smallest[0]:
+                    smallest = (held, plan)
+                if held <= budget and (best is None or plan[0] < best[0]):
+                    best = plan
+            if best is not None and (n >= n0 + 3 or best[0] == 0):
                 break
-        _, n, counts = best
+        _, n, counts = best if best is not None else smallest[1]
         spans = []
         for c, (_, rep, *_r) in zip(counts, groups):
             ends = np.cumsum([0] + list(c)) * rep
