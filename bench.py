@@ -92,6 +92,65 @@ def roofline(kernel, kernel_ms, alg_bytes):
     return out
 
 
+def sweep_roofline(acqs, rows, sweep_ms, alg_bytes):
+    """Roofline of the uneven batch's adjoint sweep (its slower direction): VALU lane-ops
+    per step from the per-pair-step instruction counts of each grid size's adjoint
+    kernel (profiles/sweep_uneven_counters.json) over the sweep's HIP-event time."""
+    t = sweep_ms * 1e-3
+    out = {'bound': 'valu', 'kernel': 'sweep_adjoint', 'achieved': None, 'peak': PEAK_VALU_TOPS, 'unit': 'TOP/s',
+           'frac': None, 'traffic': None, 'kernel_ms': sweep_ms, 'algorithmic_bytes_per_direction': alg_bytes,
+           'algorithmic_rate_TBps': alg_bytes / t / 1e12,
+           'note': 'planes held in registers and LDS for the whole record (band kernels: a team of workgroups '
+                   'per plane trading edge rows every step; column-strip kernels: one workgroup per plane), '
+                   'so the 16 B/cell-update algorithmic rate exceeds the HBM peak; VALU lane-ops per step = '
+                   'pair-steps x VALU instructions per pair-step of each size\'s adjoint kernel (counter passes), '
+                   'traffic = their measured HBM bytes per pair-step x pair-steps'}
+    try:
+        with open(os.path.join(ROOT, 'profiles', 'sweep_uneven_counters.json')) as f:
+            rec = json.load(f)['kernels']
+    except (OSError, ValueError, KeyError):
+        return out
+    names = {(70, 70): 'strip_adjoint_70x70', (100, 140): 'band_adjoint_100x140', (200, 200): 'band_adjoint_200x200'}
+    ops = traffic = 0.0
+    for a, nb in zip(acqs, rows):
+        k = rec.get(names.get((a.nz, a.nx), ''))
+        if not k:
+            return out
+        pairs = nb * a.ns * a.nt * (a.nz + 80) * (a.nx + 80) / 2
+        ops += pairs * k['valu_per_pair_step']
+        traffic += pairs * (k['hbm_read_bytes'] + k['hbm_write_bytes']) / k['pair_steps']
+    achieved = ops / t / 1e12
+    out.update(achieved=achieved, frac=achieved / PEAK_VALU_TOPS, traffic=traffic,
+               hbm={'achieved': traffic / t / 1e9, 'peak': PEAK_HBM_GBS, 'unit': 'GB/s',
+                    'frac': traffic / t / 1e9 / PEAK_HBM_GBS},
+               valu_busy={n: rec[n]['valu_busy'] for n in names.values()},
+               source='profiles/sweep_uneven_counters.json')
+    return out
+
+
+def autograd_roofline(kernel, per, alg, pairs):
+    """Roofline of the differentiation workload's slower direction: the batched workload's
+    column-strip kernels (VALU instructions per pair-step from its counter passes), the
+    forward counted twice (the segment recompute)."""
+    t = per[kernel] * 1e-3
+    out = {'bound': 'valu', 'kernel': kernel, 'achieved': None, 'peak': PEAK_VALU_TOPS, 'unit': 'TOP/s',
+           'frac': None, 'traffic': None, 'kernel_ms_per_step': per, 'algorithmic_bytes_per_direction': alg,
+           'algorithmic_rate_TBps': alg / t / 1e12,
+           'note': 'VALU lane-ops = pair-steps x VALU instructions per pair-step of the batched workload\'s '
+                   'kernels (the forward twice: forward and segment recompute); 80 propagations hold at most '
+                   '80 of the 256 CUs, which caps the fraction near 0.31.  Kernel times from one extra step with '
+                   'events around every launch (recomputes then on the caller stream; the timed steps run them '
+                   'on a side stream)'}
+    rec = counters()
+    if not rec:
+        return out
+    vp = {'forward': 2 * rec['kernels']['forward']['valu_per_pair_step'],
+          'adjoint': rec['kernels']['adjoint']['valu_per_pair_step']}
+    achieved = pairs * vp[kernel] / t / 1e12
+    out.update(achieved=achieved, frac=achieved / PEAK_VALU_TOPS, source='profiles/rtm_batched_counters.json')
+    return out
+
+
 def median_rate(units, seconds):
     return units / float(np.median(seconds))
 
@@ -340,6 +399,22 @@ def mixed_cpu_baseline(acqs, batches, shot_images):
                       f'{cores} threads; {", ".join(parts)}), extrapolated linearly to every row'}, imgs
 
 
+def mixed_kernels(ms, acqs):
+    """Kernels each grid size of a mixed sweep runs on: tiled (a tiling was chosen),
+    band (a team of workgroups per plane) or column strip (one workgroup per plane)."""
+    cus = torch.cuda.get_device_properties(ms.device).multi_processor_count
+    out = {}
+    for k, a in enumerate(acqs):
+        r = ms._round(k)
+        if any(t != (0, 0, 0) for t in ms.tiles[k].values()) or r <= 0:
+            out[f'{a.nz}x{a.nx}'] = 'tiled'
+        elif cus // r > 1:
+            out[f'{a.nz}x{a.nx}'] = f'band, {cus // r} workgroups per plane'
+        else:
+            out[f'{a.nz}x{a.nx}'] = 'column strip, one workgroup per plane'
+    return out
+
+
 def run_mixed(args, steps=None, warmup=None):
     """Uneven batch: 32 models each of 70x70 (nt 1000, 5 shots), 100x140 (nt 1500,
     7 shots) and 200x200 (nt 2000, 10 shots) x 3 backgrounds in one mixed sweep."""
@@ -369,7 +444,6 @@ def run_mixed(args, steps=None, warmup=None):
     per = {}
     for name, a, b in events:
         per[name] = per.get(name, 0.0) + a.elapsed_time(b) / steps
-    dominant = max(per, key=per.get)
     cell_updates = sum(len(b['v_bg']) * a.ns * a.nt * (a.nz + 80) * (a.nx + 80) for a, b in zip(acqs, batches))
     alg_bytes = BYTES_PER_CELL_UPDATE * cell_updates
     ms_per_step = float(np.median(step_s)) * 1e3
@@ -383,14 +457,9 @@ def run_mixed(args, steps=None, warmup=None):
                                'dx 10 m, 15 Hz Ricker', 'models_per_size': args.models_per_size,
                    'shot_images': shot_images, 'seed': args.seed, 'chunks': nchunks,
                    'tiles': {f'{a.nz}x{a.nx}': {('forward', 'adjoint')[d]: t for d, t in ms.tiles[k].items()}
-                             for k, a in enumerate(acqs)}},
-        'roofline': {'bound': 'hbm', 'kernel': dominant, 'achieved': alg_bytes / (per[dominant] * 1e-3) / 1e9,
-                     'peak': PEAK_HBM_GBS, 'unit': 'GB/s', 'frac': None, 'traffic': None,
-                     'algorithmic_rate_frac': alg_bytes / (per[dominant] * 1e-3) / 1e9 / PEAK_HBM_GBS,
-                     'sweep_ms_per_step': per, 'algorithmic_bytes_per_direction': alg_bytes,
-                     'note': 'tiled kernels: state in LDS/registers within a launch, in HBM between launches; '
-                             'algorithmic 16 B per cell-update rate (measured traffic and issue in '
-                             'profiles/sweep_uneven_kernel_trace.md)'},
+                             for k, a in enumerate(acqs)},
+                   'kernels': mixed_kernels(ms, acqs)},
+        'roofline': dict(sweep_roofline(acqs, rows, per['sweep_adjoint'], alg_bytes), sweep_ms_per_step=per),
     }
     if not args.no_cpu:
         base, imgs = mixed_cpu_baseline(acqs, batches, shot_images)
@@ -448,13 +517,7 @@ def run_autograd(args, steps=None, warmup=None):
         'config': {'workload': 'differentiation: misfit gradient of 16 mixed 70x70 models x 5 shots, nt 1000, '
                                'autograd.Function, checkpoints every 50 steps with recompute',
                    'models': len(true), 'seed': args.seed, 'misfit': loss},
-        'roofline': {'bound': 'hbm', 'kernel': dominant, 'achieved': alg / (per[dominant] * 1e-3) / 1e9,
-                     'peak': PEAK_HBM_GBS, 'unit': 'GB/s', 'frac': None, 'traffic': None,
-                     'algorithmic_rate_frac': alg / (per[dominant] * 1e-3) / 1e9 / PEAK_HBM_GBS,
-                     'kernel_ms_per_step': per, 'algorithmic_bytes_per_direction': alg,
-                     'note': 'forward time includes the per-segment recompute (two forward passes); kernel times '
-                             'from one extra step with events around every launch (recomputes then on the '
-                             'caller stream; the timed steps run them on a side stream)'},
+        'roofline': autograd_roofline(dominant, per, alg, units * sim.geom.nt * sim.H * sim.W / 2),
     }
     if not args.no_cpu:
         from oracle import fd

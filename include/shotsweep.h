@@ -337,7 +337,13 @@ int ss_velocity_scores_f64(int B, int H, int W, const double *pred, const double
  * a batch on its own padded grid with its own step count, shot count and
  * source columns, advance together: one tiled, temporally blocked launch per
  * step range covers every active (propagation, tile) item of every group,
- * and the launch sequence is replayed from a cached hipGraph.  Per cell the
+ * and the launch sequence is replayed from a cached hipGraph.  A group whose
+ * tiling is left at 0 and whose call covers the whole record from zero state
+ * runs instead with its planes held on chip for the whole record: on the band
+ * kernels (one cooperative launch; see ss_sweep_round_f32) for the 200 x 200
+ * and 100 x 140 models with pad 40 and receivers on row 1, or on the
+ * column-strip kernels (one workgroup per propagation; its state then stays in
+ * buffer `phase`) for 70 x 70 with pad 40 and row 1.  Per cell the
  * arithmetic is the reference's, as in ss_forward_f32 / ss_adjoint_f32 over
  * [0, nt) with the whole Laplacian history:
  *   forward  gpu.py:111-115 and 126-135 (gathers, Laplacian history)
