@@ -420,6 +420,17 @@ int ss_sweep_autotune_f32(int ngroups, ss_sweep_group *groups, int direction, in
  */
 int ss_sweep_round_f32(const ss_geom *geom);
 
+/*
+ * Count of band-kernel exchange waits on the current device that ran past their
+ * bound (about a second of polls; the environment variable SHOTSWEEP_BAND_SPIN
+ * sets the number of polls) since the library was loaded.  Such a wait fills the
+ * band's halo with NaN and lets the launch finish, so the failure shows in the
+ * outputs; with every workgroup of a team resident (a cooperative launch) it does
+ * not occur.  Synchronous.  No counterpart in the reference (diagnostics of the
+ * band kernels).
+ */
+int ss_sweep_band_errors(int *count);
+
 #ifdef __cplusplus
 }
 #endif

@@ -12,7 +12,7 @@ LIB_PATH = Path(__file__).resolve().parent / 'libshotsweep.so'
 MAX_SHOTS = 32
 IMPL_AUTO, IMPL_STEPWISE, IMPL_RESIDENT, IMPL_TILED, IMPL_UNTILED = 0, 1, 2, 3, 4
 RTM_FORWARD, RTM_ADJOINT = 1, 2
-ABI_VERSION = 7
+ABI_VERSION = 8
 
 _vp = ctypes.c_void_p
 _i = ctypes.c_int
@@ -92,6 +92,7 @@ _SWEEP = {
     'ss_sweep_adjoint_f32': [_i, ctypes.POINTER(SSSweepGroup), _i, _vp],
     'ss_sweep_autotune_f32': [_i, ctypes.POINTER(SSSweepGroup), _i, _i, _vp],
     'ss_sweep_round_f32': [ctypes.POINTER(SSGeom)],
+    'ss_sweep_band_errors': [ctypes.POINTER(_i)],
 }
 
 _RTM = {

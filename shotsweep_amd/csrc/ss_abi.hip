@@ -289,7 +289,7 @@ extern "C" {
 
 const char *ss_last_error(void) { return ss::g_last_error.c_str(); }
 
-int ss_version(void) { return 7; }
+int ss_version(void) { return 8; }
 
 int64_t ss_resident_max_cells(void) { return resident_max_cells(); }
 
@@ -524,6 +524,19 @@ int ss_sweep_autotune_f32(int ngroups, ss_sweep_group *groups, int direction, in
     return sweep_autotune(ngroups, groups, direction, max_trials, (hipStream_t)stream);
   } catch (const std::exception &e) {
     set_error(std::string("sweep autotune: ") + e.what());
+    return SS_EHIP;
+  }
+}
+
+int ss_sweep_band_errors(int *count) {
+  if (!count) {
+    set_error("sweep band errors: count is required");
+    return SS_EINVAL;
+  }
+  try {
+    return band_errors(count);
+  } catch (const std::exception &e) {
+    set_error(std::string("sweep band errors: ") + e.what());
     return SS_EHIP;
   }
 }

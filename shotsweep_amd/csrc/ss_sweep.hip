@@ -28,6 +28,7 @@
 // forward gpu.py:111-115 and 126-135, adjoint gpu.py:177-184.
 #include <algorithm>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <map>
@@ -778,8 +779,16 @@ std::string tune_key(int dir, int H, int W, int pad, int64_t nprop) {
 }
 
 // Polls of a band exchange before it counts as failed: about a second, against a
-// few microseconds per exchange when every workgroup of a team is resident.
+// few microseconds per exchange when every workgroup of a team is resident.  The
+// environment variable SHOTSWEEP_BAND_SPIN overrides it (a test forces the failure
+// path with 0: the launch still ends, the missed halos become NaN).
 constexpr int kBandSpin = 1 << 20;
+int band_spin() {
+  const char *e = std::getenv("SHOTSWEEP_BAND_SPIN");
+  if (!e || !*e) return kBandSpin;
+  const long v = std::strtol(e, nullptr, 10);
+  return v < 0 ? 0 : (v > kBandSpin ? kBandSpin : (int)v);
+}
 
 // A whole-record call from zero state with no tiling asked for (the full-history
 // sweeps, which run alone on their stream).
@@ -1249,7 +1258,7 @@ int sweep_run(int ngroups, const ss_sweep_group *groups, int dir, int use_graph,
   for (int i = 0; i < ngroups; ++i) {
     Geom g;
     if (use_band(groups[i], dir, &g)) {
-      const int rc = band_run(groups[i], g, dir, kBandSpin, st);
+      const int rc = band_run(groups[i], g, dir, band_spin(), st);
       if (rc != SS_OK) return rc;
     } else if (use_resident(groups[i], &g)) {
       const int rc = resident_run(groups[i], g, dir, st);

@@ -8,6 +8,8 @@ plane and one step per launch.  The stepwise path itself is pinned to the
 reference's golden vectors (test_gpu_parity.py); the images are also checked
 against the oracle at 1e-5 relative L2.
 """
+import ctypes
+
 import numpy as np
 import pytest
 import torch
@@ -323,3 +325,45 @@ def test_band_adjoint_bit_identical_to_stepwise(shape):
         if tiles == (0, 0, 0):  # one launch from state buffer 0: the final state in buffer 1
             assert torch.equal(state[2], m1) and torch.equal(state[3], m2)
         print(f'  tiles {tiles}: equal', flush=True)
+
+
+def test_band_wait_bound_fails_into_nan_not_a_hang(monkeypatch):
+    """The band kernels' exchange waits are bounded: with the bound forced to zero polls
+    from the host (SHOTSWEEP_BAND_SPIN=0) the launch still ends, waits that miss their
+    words count as errors (ss_sweep_band_errors) and fill the band's halo with NaN, so
+    the gathers show it; the next call with the default bound is exact again (the
+    exchange words carry a launch epoch, so nothing of the failed launch is reused)."""
+    acq = Acquisition(nz=200, nx=200, nt=60, ns=10)
+    case(f'band wait bound forced to zero, {acq.nz}x{acq.nx} nt {acq.nt} x 2 rows')
+    sim = gpu.Sim(geom=acq)
+    nb, g = 2, acq
+    prep = sim._prepare(models(acq, nb, 13))
+    H, W = sim.H, sim.W
+    ref_out, ref_hist = stepwise_forward(sim, prep)
+    lib = _lib.load()
+    count = ctypes.c_int(0)
+
+    def errors():
+        _lib.check(lib.ss_sweep_band_errors(ctypes.byref(count)), 'band errors')
+        return count.value
+
+    def run():
+        state = torch.zeros(4, nb, g.ns, H, W, device=DEV)
+        hist = torch.empty_like(ref_hist)
+        out = torch.empty_like(ref_out)
+        arr = _forward_group(sim, prep, nb, state, hist, out)
+        _lib.check(lib.ss_sweep_forward_f32(1, arr, 1, _lib.stream_handle(DEV)), 'sweep forward')
+        torch.cuda.synchronize()
+        return out, hist
+
+    before = errors()
+    monkeypatch.setenv('SHOTSWEEP_BAND_SPIN', '0')
+    out, _ = run()
+    failed = errors() - before
+    print(f'  waits past the bound: {failed}, NaN gathers: {int(torch.isnan(out).sum())}', flush=True)
+    assert failed > 0 and bool(torch.isnan(out).any())
+    monkeypatch.delenv('SHOTSWEEP_BAND_SPIN')
+    before = errors()
+    out, hist = run()
+    assert errors() == before
+    assert torch.equal(out, ref_out) and torch.equal(hist, ref_hist)
