@@ -75,7 +75,7 @@ int64_t rtm_workspace_bytes(const ss_geom *g, int segment, int elem_size);
 int sweep_run(int ngroups, const ss_sweep_group *groups, int dir, int use_graph, hipStream_t st);
 int sweep_autotune(int ngroups, ss_sweep_group *groups, int dir, int max_trials, hipStream_t st);
 bool sweep_tuned(int dir, int H, int W, int pad, int64_t nprop);  // tiling tuned for this grid and propagation count
-int sweep_launches(const ss_sweep_group &group, int dir);  // launches of one call (< 0: error)
+int sweep_launches(const ss_sweep_group &group, int dir, hipStream_t st);  // launches of one call on st (< 0: error)
 bool sweep_fits(const Geom &g);                             // within the tiled kernels' limits
 
 // Band kernels (ss_band.hip): planes too large for one workgroup held by a team of

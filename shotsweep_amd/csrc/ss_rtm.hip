@@ -322,7 +322,7 @@ ss_sweep_group tiled_group(const ss_geom &gg, const Layout<float> &L, const floa
 }
 
 int tiled_run(const ss_sweep_group &q, int dir, hipStream_t st, int *launches, bool use_graph) {
-  const int n = sweep_launches(q, dir);
+  const int n = sweep_launches(q, dir, st);
   if (n < 0) return n;
   *launches = n;
   size_t m = 0;

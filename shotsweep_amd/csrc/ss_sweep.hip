@@ -838,6 +838,13 @@ int resident_run(const ss_sweep_group &in, const Geom &g, int dir, hipStream_t s
                           static_cast<const float *>(in.hist), s0, s1, gacc, illum, sg, st, nullptr);
 }
 
+// A stream being captured into a graph cannot take a cooperative launch with its
+// co-residency guarantee: captured calls keep to the tiled kernels.
+bool capturing(hipStream_t st) {
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  return hipStreamIsCapturing(st, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone;
+}
+
 // A group runs on the band kernels when a band layout exists for its grid, the
 // caller asks for no particular tiling, and the call covers the whole record from
 // zero state (the full-history sweeps, which run alone on their stream).
@@ -1255,9 +1262,10 @@ int sweep_run(int ngroups, const ss_sweep_group *groups, int dir, int use_graph,
   }
   // band groups run whole in one cooperative launch each, the others in the tiled launches
   std::vector<ss_sweep_group> tiled;
+  const bool cap = capturing(st);
   for (int i = 0; i < ngroups; ++i) {
     Geom g;
-    if (use_band(groups[i], dir, &g)) {
+    if (!cap && use_band(groups[i], dir, &g)) {
       const int rc = band_run(groups[i], g, dir, band_spin(), st);
       if (rc != SS_OK) return rc;
     } else if (use_resident(groups[i], &g)) {
@@ -1279,9 +1287,9 @@ bool sweep_tuned(int dir, int H, int W, int pad, int64_t nprop) {
   return g_tuned.count(tune_key(dir, H, W, pad, nprop)) != 0;
 }
 
-int sweep_launches(const ss_sweep_group &group, int dir) {
+int sweep_launches(const ss_sweep_group &group, int dir, hipStream_t st) {
   Geom g;
-  if (use_band(group, dir, &g)) return 1;
+  if (!capturing(st) && use_band(group, dir, &g)) return 1;
   if (use_resident(group, &g)) return 0;  // state read and written in place: buffer `phase`
   SGroup o;
   char err[256] = {0};

@@ -367,3 +367,36 @@ def test_band_wait_bound_fails_into_nan_not_a_hang(monkeypatch):
     out, hist = run()
     assert errors() == before
     assert torch.equal(out, ref_out) and torch.equal(hist, ref_hist)
+
+
+def test_captured_sweep_keeps_to_tiled_kernels():
+    """A sweep call captured into a graph on a band grid (200 x 200 models, pad 40)
+    runs the tiled kernels, since a captured cooperative launch would lose its
+    co-residency guarantee: three replays, with eager work and synchronisations in
+    between, each give the gathers and history of the eager band call bit for bit."""
+    acq = Acquisition(nz=200, nx=200, nt=60, ns=10)
+    case(f'captured sweep {acq.nz}x{acq.nx} nt {acq.nt} x 1 row')
+    sim = gpu.Sim(geom=acq)
+    nb, g = 1, acq
+    prep = sim._prepare(models(acq, nb, 14))
+    H, W = sim.H, sim.W
+    lib = _lib.load()
+    state = torch.zeros(4, nb, g.ns, H, W, device=DEV)
+    ref_hist = torch.empty(nb, g.ns, g.nt, H, W, device=DEV)
+    ref_out = torch.empty(nb, g.ns, g.nt, g.nx, device=DEV)
+    arr = _forward_group(sim, prep, nb, state, ref_hist, ref_out)
+    _lib.check(lib.ss_sweep_forward_f32(1, arr, 0, _lib.stream_handle(DEV)), 'eager sweep')
+    torch.cuda.synchronize()
+    hist, out = torch.full_like(ref_hist, float('nan')), torch.full_like(ref_out, float('nan'))
+    arr = _forward_group(sim, prep, nb, state, hist, out)
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        _lib.check(lib.ss_sweep_forward_f32(1, arr, 0, _lib.stream_handle(DEV)), 'captured sweep')
+    for rep in range(3):
+        hist.fill_(float('nan'))
+        out.fill_(float('nan'))
+        x = torch.randn(1000, device=DEV)
+        _ = float((x * x).sum())      # eager work between replays
+        graph.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(out, ref_out) and torch.equal(hist, ref_hist), f'replay {rep}'
