@@ -288,17 +288,19 @@ def _adjoint_group(sim, prep, nb, state, res, hist, gacc, illum, sg, tiles=(0, 0
     return (_lib.SSSweepGroup * 1)(g)
 
 
-@pytest.mark.parametrize('shape', [(200, 200, 10, 7, (70, 94, 6)), (100, 140, 7, 19, (90, 74, 6))])
+@pytest.mark.parametrize('shape', [(200, 200, 10, 7, (70, 94, 6), 300), (100, 140, 7, 19, (90, 74, 6), 301)])
 def test_band_adjoint_bit_identical_to_stepwise(shape):
     """The adjoint on the band kernels (the bands trade the new state of their edge
     rows every step and form the halo's q from it) at the same two shapes and
-    propagation counts as the forward test, with the stepwise forward's Laplacian
-    history and its gathers as the residual: imaging sum, illumination, source term
-    and the final state (m1, m2) equal the stepwise adjoint's bit for bit; the state
-    buffers and outputs are NaN on entry.  The tiled kernels under a forced tiling
-    give the same imaging sum, illumination and source term."""
-    nz, nx, ns, nb, tiling = shape
-    acq = Acquisition(nz=nz, nx=nx, nt=300, ns=ns)
+    propagation counts as the forward test, nt 300 and 301 (an odd step count: the
+    time loop's last step outside its pairs), with
+    the stepwise forward's Laplacian history and its gathers as the residual:
+    imaging sum, illumination, source term and the final state (m1, m2) equal the
+    stepwise adjoint's bit for bit; the state buffers and outputs are NaN on
+    entry.  The tiled kernels under a forced tiling give the same imaging sum,
+    illumination and source term."""
+    nz, nx, ns, nb, tiling, nt = shape
+    acq = Acquisition(nz=nz, nx=nx, nt=nt, ns=ns)
     case(f'band adjoint {acq.nz}x{acq.nx} nt {acq.nt} ns {acq.ns} x {nb} rows vs stepwise')
     sim = gpu.Sim(geom=acq)
     g = acq
@@ -367,6 +369,51 @@ def test_band_wait_bound_fails_into_nan_not_a_hang(monkeypatch):
     out, hist = run()
     assert errors() == before
     assert torch.equal(out, ref_out) and torch.equal(hist, ref_hist)
+
+
+def test_band_adjoint_wait_bound_fails_into_nan_not_a_hang(monkeypatch):
+    """The adjoint band kernel's exchange waits are bounded as the forward's: with the
+    bound forced to zero polls from the host the launch ends, the missed waits count
+    as errors and the halo's NaN reaches the imaging sum; the next call with the
+    default bound equals the stepwise adjoint bit for bit."""
+    acq = Acquisition(nz=200, nx=200, nt=61, ns=10)
+    case(f'band adjoint wait bound forced to zero, {acq.nz}x{acq.nx} nt {acq.nt} x 2 rows')
+    sim = gpu.Sim(geom=acq)
+    nb, g = 2, acq
+    prep = sim._prepare(models(acq, nb, 14))
+    H, W = sim.H, sim.W
+    res, hist = stepwise_forward(sim, prep)
+    z = lambda *s: torch.zeros(*s, device=DEV)
+    m1, m2 = z(nb, g.ns, H, W), z(nb, g.ns, H, W)
+    ref_gacc, ref_illum, ref_sg = z(nb, g.ns, H, W), z(nb, g.ns, g.nz, g.nx), z(nb, g.ns)
+    sim._adjoint(sim._geom(nb), 0, g.nt, prep['c2'], prep['sd'], res, hist, m1, m2, ref_gacc, ref_illum, ref_sg,
+                 impl=_lib.IMPL_STEPWISE)
+    lib = _lib.load()
+    count = ctypes.c_int(0)
+
+    def errors():
+        _lib.check(lib.ss_sweep_band_errors(ctypes.byref(count)), 'band errors')
+        return count.value
+
+    def run():
+        state = torch.zeros(4, nb, g.ns, H, W, device=DEV)
+        gacc, illum, sg = torch.empty_like(ref_gacc), torch.empty_like(ref_illum), torch.empty_like(ref_sg)
+        arr = _adjoint_group(sim, prep, nb, state, res, hist, gacc, illum, sg)
+        _lib.check(lib.ss_sweep_adjoint_f32(1, arr, 1, _lib.stream_handle(DEV)), 'sweep adjoint')
+        torch.cuda.synchronize()
+        return gacc, illum, sg
+
+    before = errors()
+    monkeypatch.setenv('SHOTSWEEP_BAND_SPIN', '0')
+    gacc, _, _ = run()
+    failed = errors() - before
+    print(f'  waits past the bound: {failed}, NaN imaging-sum cells: {int(torch.isnan(gacc).sum())}', flush=True)
+    assert failed > 0 and bool(torch.isnan(gacc).any())
+    monkeypatch.delenv('SHOTSWEEP_BAND_SPIN')
+    before = errors()
+    gacc, illum, sg = run()
+    assert errors() == before
+    assert torch.equal(gacc, ref_gacc) and torch.equal(illum, ref_illum) and torch.equal(sg, ref_sg)
 
 
 def test_captured_sweep_keeps_to_tiled_kernels():
