@@ -239,19 +239,20 @@ def _forward_group(sim, prep, nb, state, hist, gathers, tiles=(0, 0, 0)):
     return (_lib.SSSweepGroup * 1)(g)
 
 
-@pytest.mark.parametrize('shape', [(200, 200, 10, 7, (94, 94, 8)), (100, 140, 7, 19, (60, 110, 6))])
+@pytest.mark.parametrize('shape', [(200, 200, 10, 7, (94, 94, 8), 300), (100, 140, 7, 19, (60, 110, 6), 301)])
 def test_band_forward_bit_identical_to_stepwise(shape):
     """The band kernels, which hold a plane in a team of workgroups that trade
     boundary rows every step: 280 x 280 planes (200 x 200 models, pad 40) in four
     bands, 7 rows x 10 shots = 70 propagations, two rounds of the 64 teams; 180 x
     220 planes (100 x 140 models) in two bands, 19 rows x 7 shots = 133
-    propagations, two rounds of the 128 teams (the second rounds partial).
+    propagations, two rounds of the 128 teams (the second rounds partial); nt 300
+    and 301 (an odd step count: the time loop's last step outside its pairs).
     Gathers, the whole Laplacian history (zero on the rim) and the final state
     equal the stepwise kernels' bit for bit; the state buffers are NaN on entry,
     which a zero-state call must not read.  The tiled kernels under a forced
     tiling of the same grid give the same gathers and history."""
-    nz, nx, ns, nb, tiling = shape
-    acq = Acquisition(nz=nz, nx=nx, nt=300, ns=ns)
+    nz, nx, ns, nb, tiling, nt = shape
+    acq = Acquisition(nz=nz, nx=nx, nt=nt, ns=ns)
     case(f'band forward {acq.nz}x{acq.nx} nt {acq.nt} ns {acq.ns} x {nb} rows vs stepwise')
     sim = gpu.Sim(geom=acq)
     g = acq
