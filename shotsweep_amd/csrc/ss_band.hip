@@ -93,6 +93,11 @@ struct BandArgs {
 // tag matches the step it needs, so no separate counter, and no wait for the
 // stores to complete on the writer's side, sits on the path between two bands.
 typedef uint64_t *XR;  // a team's exchange region
+// Pause of the exchange waves between polling rounds (units of 64 clocks, at most
+// 127): polls share the memory system with the history stream, and on the uneven
+// batch (same device, alternating builds) the forward's step is shortest at 32
+// (1 and 4: +1.5 %, 64: +5 %) and the adjoint's at 96 (1: +3.5 %, 127: +2.5 %).
+constexpr int kFwdSleep = 32, kAdjSleep = 96;
 __device__ __forceinline__ void x_put(uint64_t *p, float v, int tag) {
   const uint64_t u = ((uint64_t)(uint32_t)tag << 32) | __builtin_bit_cast(uint32_t, v);
   __hip_atomic_store(p, u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -114,10 +119,11 @@ __device__ __forceinline__ XR x_rsrc(uint64_t *xbuf, int team) { return xbuf + (
 // The exchange wave: the neighbours' rows of p(t) (tag) into the halo rows, LDS
 // rows 0, 1 (from the band above) and BH + 2, BH + 3 (from the band below).  Every
 // lane loads its words of both sides at once, keeps those whose tag matches and
-// reloads the rest; past spin_limit rounds the missing words become NaN and the
-// error counter rises.  `conv` turns a received word into the halo's value (side,
-// word index, value).
-template <class S, class Conv>
+// reloads the rest, sleeping SLEEP x 64 clocks between rounds (fewer polls in
+// the memory system under the history stream); past spin_limit rounds the
+// missing words become NaN and the error counter rises.  `conv` turns a received
+// word into the halo's value (side, word index, value).
+template <class S, int SLEEP, class Conv>
 __device__ void x_receive_sides(XR xr, int band, int tag, uint32_t halo_addr, int lane, int spin_limit, int *err,
                                 bool &failed, Conv conv) {
   constexpr int NB = S::H / S::BH, NW = S::XSIDE, NWL = (NW + 63) / 64;
@@ -160,14 +166,14 @@ __device__ void x_receive_sides(XR xr, int band, int tag, uint32_t halo_addr, in
           if ((pending >> (sd * NWL + k)) & 1u) lds_st1(dst[sd] + 4u * (uint32_t)(lane + 64 * k), __builtin_nanf(""));
       break;
     }
-    __builtin_amdgcn_s_sleep(1);
+    __builtin_amdgcn_s_sleep(SLEEP);
   }
 }
 
 template <class S>
 __device__ void x_receive(XR xr, int band, int tag, uint32_t band_addr, int lane, int spin_limit, int *err,
                           bool &failed) {
-  x_receive_sides<S>(xr, band, tag, band_addr, lane, spin_limit, err, failed,
+  x_receive_sides<S, kFwdSleep>(xr, band, tag, band_addr, lane, spin_limit, err, failed,
                      [](int, int, float v) { return v; });
 }
 
@@ -442,7 +448,7 @@ template <class S>
 __device__ void x_receive_q(XR xr, int band, int tag, uint32_t q_addr, uint32_t coef_addr, int lane, int spin_limit,
                             int *err, bool &failed) {
   constexpr uint32_t kRow = 4u * S::W;
-  x_receive_sides<S>(xr, band, tag, q_addr, lane, spin_limit, err, failed, [&](int sd, int w, float m) {
+  x_receive_sides<S, kAdjSleep>(xr, band, tag, q_addr, lane, spin_limit, err, failed, [&](int sd, int w, float m) {
     const float ca = lds_table_a(coef_addr + (uint32_t)sd * 2u * kRow + 4u * (uint32_t)w);
     const float q = div12(ca * m);
     return ca != 0.0f ? q : 0.0f;
