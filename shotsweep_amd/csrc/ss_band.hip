@@ -94,9 +94,9 @@ struct BandArgs {
 // stores to complete on the writer's side, sits on the path between two bands.
 typedef uint64_t *XR;  // a team's exchange region
 // Pause of the exchange waves between polling rounds (units of 64 clocks, at most
-// 127): polls share the memory system with the history stream, and on the uneven
-// batch (same device, alternating builds) the forward's step is shortest at 32
-// (1 and 4: +1.5 %, 64: +5 %) and the adjoint's at 96 (1: +3.5 %, 127: +2.5 %).
+// 127): polls share the memory system with the history stream.  Measured on the
+// uneven batch: the adjoint's sweep 3.5 % shorter at 96 than at 1 (127: slower
+// again); the forward's no better than at 1 between 16 and 32 and slower at 64.
 constexpr int kFwdSleep = 32, kAdjSleep = 96;
 __device__ __forceinline__ void x_put(uint64_t *p, float v, int tag) {
   const uint64_t u = ((uint64_t)(uint32_t)tag << 32) | __builtin_bit_cast(uint32_t, v);
