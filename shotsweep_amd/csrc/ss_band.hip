@@ -252,6 +252,7 @@ __global__ __launch_bounds__(kBandThreads) void fwd_band_kernel(BandArgs a) {
   // reads, and never rows a neighbour may still be reading.
   auto tag0 = [&](int prop) { return (a.epoch << 20) + (prop / nteams) * (nslots + 2); };
   if (xw) {  // the exchange wave: one receive per step, the compute waves' barriers
+    __builtin_amdgcn_s_setprio(2);  // its halo work ahead of the compute waves' issue (uneven step -1 %)
     bool failed = false;
     for (int prop = team; prop < a.nprop; prop += nteams) {
       const int gbase = tag0(prop);
@@ -527,6 +528,7 @@ __global__ __launch_bounds__(kBandThreads) void adj_band_kernel(BandArgs a) {
     }
   };
   if (xw) {  // the exchange wave: one receive per step, the compute waves' barriers
+    __builtin_amdgcn_s_setprio(2);  // its halo work ahead of the compute waves' issue (uneven step -1 %)
     bool failed = false;
     for (int prop = team; prop < a.nprop; prop += nteams) {
       const int gbase = tag0(prop);
